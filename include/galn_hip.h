@@ -1,0 +1,161 @@
+/* galn_hip.h -- C ABI of the MI355X (gfx950) BWA-aln backtracking search.
+ *
+ * The library searches short reads against a BWA .bwt index resident in HBM
+ * and returns, per read, the SA intervals (.sai records) that BWA aln's
+ * bwt_match_gap finds, byte-identical to the reference's CPU path.
+ *
+ * Plain pointers and sizes only. Host buffers are caller-owned; device
+ * buffers (index, reads, arenas, queues) are owned by the library handles.
+ * Every call returns 0 or a negative GALN_HIP_E_* code. Calls on one index
+ * handle are serialised by the caller; the serving API is thread-safe.
+ *
+ * Interfaces this replaces in the reference (bilalbioinfo/galn 0.2.7):
+ *   galn_hip_index_open/close   index upload of run_device, src/galn.cu:804-825, :1190-1192
+ *   galn_hip_search_plain       aln_search_host (src/aln_host.h:46-48) per read, and the
+ *                               light tier launch light_kernel, src/galn.cu:835-905
+ *   galn_hip_search_batched     phase W/G persistent warp engine gapw_kernel with its
+ *                               device work list, src/gapw.cuh:716-795, src/galn.cu:907-1180
+ *   galn_hip_readset_*          the device read arrays of run_device, src/galn.cu:780-825
+ *   galn_hip_server_*           no counterpart (the reference is a one-shot CLI); serving loop
+ *                               with the index resident in HBM and continuous batching
+ */
+#ifndef GALN_HIP_H
+#define GALN_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define GALN_HIP_ABI_VERSION 1
+
+/* ---- library return codes ---- */
+#define GALN_HIP_OK             0
+#define GALN_HIP_E_ARG        (-1)   /* bad argument */
+#define GALN_HIP_E_DEVICE     (-2)   /* no such device / HIP runtime error */
+#define GALN_HIP_E_NOMEM      (-3)   /* device memory exhausted */
+#define GALN_HIP_E_UNSUPPORTED (-4)  /* a read or option outside the kernel's limits */
+#define GALN_HIP_E_RECCAP     (-5)   /* caller's record array too small: out->n_rec says how many */
+#define GALN_HIP_E_BUSY       (-6)   /* serving: admission limit reached */
+
+/* ---- per-read result codes (out->n_aln[i] < 0), as src/bwtgap_hd.h:41-43 ---- */
+#define GALN_READ_ERR_SCORE   (-3)   /* a push with score outside [0, n_stacks): final */
+#define GALN_READ_ERR_UNSUP   (-5)   /* read outside the kernel's limits (see galn_hip_limits) */
+
+/* search flags */
+#define GALN_HIP_TRACE   0x1u        /* compute the per-read trace hash (pop-order check) */
+#define GALN_HIP_NOFUSE  0x2u        /* never keep the exact child in registers (testing) */
+
+/* Index descriptor: field order of bwtgpu_meta_t, src/bwtsearch.h:26-31. */
+typedef struct {
+    uint64_t primary;     /* row of $ */
+    uint64_t L2[5];       /* C(): L2[0] = 0, L2[4] = seq_len */
+    uint64_t seq_len;     /* BWT length without $ */
+    uint64_t bwt_size;    /* u32 words of the .bwt body */
+} galn_bwt_meta_t;
+
+/* Search options of one bwa read batch: field order of bwtgap_opt_t, src/bwtgap_hd.h:52-59.
+ * max_diff is the batch value; the per-read value comes in galn_hip_reads.max_diff. */
+typedef struct {
+    int32_t s_mm, s_gapo, s_gape;
+    int32_t mode;
+    int32_t indel_end_skip, max_del_occ, max_entries;
+    int32_t max_diff, max_gapo, max_gape;
+    int32_t max_seed_diff, seed_len;
+    int32_t max_top2;
+} galn_search_opt_t;
+
+/* One .sai record (bwt_aln1_t, 24 bytes): packed = n_mm:8 n_gapo:8 n_gape:8 score:20
+ * n_ins:10 n_del:10 (LSB first), then the SA interval [k, l]. src/sai_format.h:30-45. */
+typedef struct {
+    uint64_t packed;
+    uint64_t k, l;
+} galn_aln_rec_t;
+
+/* Reads as structure-of-arrays. query: concatenated search queries, each
+ * complement(nt4(reverse(read))) with 0..3 for A,C,G,T and 4 for anything else
+ * (src/host_io.c:178-182). */
+typedef struct {
+    int32_t n_reads;
+    const uint8_t *query;
+    const uint64_t *qoff;      /* [n_reads] offset of each query in `query` */
+    const int32_t *qlen;       /* [n_reads] >= 1 */
+    const int32_t *max_diff;   /* [n_reads] per-read max_diff (aln_read_maxdiff) */
+    const int32_t *batch;      /* [n_reads] index into bopts / n_stacks */
+    int32_t n_batches;
+    const galn_search_opt_t *bopts;   /* [n_batches] (aln_batch_opt) */
+    const int32_t *n_stacks;          /* [n_batches] */
+    int32_t seed_len;          /* the -l value of the run (gap_opt_t.seed_len) */
+} galn_hip_reads;
+
+/* Results, caller-owned. Records of read i are rec[rec_off[i] .. rec_off[i] + n_aln[i]). */
+typedef struct {
+    int32_t *n_aln;            /* [n_reads] record count, or a GALN_READ_ERR_* code */
+    uint64_t *rec_off;         /* [n_reads] */
+    galn_aln_rec_t *rec;       /* [rec_cap] */
+    uint64_t rec_cap;
+    uint64_t n_rec;            /* out: records written (or needed, with GALN_HIP_E_RECCAP) */
+    uint64_t *n_pops;          /* optional [n_reads]: popped entries */
+    uint64_t *trace;           /* optional [n_reads]: trace hash (needs GALN_HIP_TRACE) */
+} galn_hip_out;
+
+/* Kernel limits; reads beyond them get GALN_READ_ERR_UNSUP. */
+typedef struct {
+    int32_t max_read_len;      /* 1023 */
+    int32_t max_n_stacks;      /* 256 */
+    int32_t max_diff;          /* 127 */
+    int32_t max_gapo_plus_gape;/* 255 */
+    uint64_t max_seq_len;      /* 2^38 - 1 */
+} galn_hip_limits_t;
+
+/* Tuning of the batched form (0 = library default / autotuned). */
+typedef struct {
+    int32_t block_threads;     /* threads per workgroup (64 or 128) */
+    int32_t waves_per_cu;      /* persistent grid size = waves_per_cu * CUs */
+    int32_t chunks_per_lane;   /* stack chunks (16 entries each) per lane in the first tier */
+    int32_t rec_per_lane;      /* record slots per lane in the first tier */
+    int32_t sort_by_cost;      /* 1: claim reads longest-first */
+} galn_hip_tune_t;
+
+typedef struct galn_hip_index galn_hip_index;
+typedef struct galn_hip_readset galn_hip_readset;
+typedef struct galn_hip_server galn_hip_server;
+
+int galn_hip_abi_version(void);
+void galn_hip_get_limits(galn_hip_limits_t *lim);
+/* Last error text of this thread (static storage). */
+const char *galn_hip_last_error(void);
+
+/* Upload a .bwt body (bwt_size words, as load_bwt returns it) to `device`. */
+int galn_hip_index_open(const uint32_t *bwt, const galn_bwt_meta_t *meta, int device,
+                        galn_hip_index **out);
+int galn_hip_index_close(galn_hip_index *idx);
+int galn_hip_index_device(const galn_hip_index *idx);
+
+/* Plain form: reads uploaded, one launch per 0x40000 reads (thread per read), re-runs of
+ * reads that outgrow the first tier in larger tiers, results copied to `out`. */
+int galn_hip_search_plain(galn_hip_index *idx, const galn_hip_reads *reads, galn_hip_out *out,
+                          uint32_t flags);
+
+/* Batched form: one persistent launch over all reads with a device work queue
+ * (lanes claim the next read as soon as theirs is done). Same results. */
+int galn_hip_search_batched(galn_hip_index *idx, const galn_hip_reads *reads, galn_hip_out *out,
+                            uint32_t flags, const galn_hip_tune_t *tune);
+
+/* Device-resident reads: upload once, search many times (benchmarks, serving). */
+int galn_hip_readset_upload(galn_hip_index *idx, const galn_hip_reads *reads, galn_hip_readset **out);
+int galn_hip_readset_free(galn_hip_readset *rs);
+/* form: 0 = plain, 1 = batched. Results stay on the device until galn_hip_readset_results. */
+int galn_hip_readset_search(galn_hip_index *idx, galn_hip_readset *rs, int form, uint32_t flags,
+                            const galn_hip_tune_t *tune);
+int galn_hip_readset_results(galn_hip_readset *rs, galn_hip_out *out);
+/* Time of the last search's kernels in ms, measured with HIP events on the
+ * search stream (first tier / all tiers). */
+int galn_hip_readset_kernel_ms(const galn_hip_readset *rs, double *first_tier_ms, double *total_ms);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif
