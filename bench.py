@@ -1,0 +1,241 @@
+#!/usr/bin/env python3
+"""bench.py -- reads/s of the MI355X BWA-aln search, batched vs plain form, on one GPU.
+
+Workload (default): the batched aDNA mixed-length configuration -- a 100 Mbp synthetic
+reference (41% GC, 10% duplicated 300 bp-5 kb segments at 0-5% divergence, 1% N runs)
+and 10M reads of log-normal length (median 45, clipped 30-150) with 5' C->T / 3' G->A
+deamination, 0.5% substitutions and 5% reads from an unrelated random sequence,
+searched with `aln -l 1024 -n 0.01 -o 2`.
+
+A step is one search of the whole read set with the reads and the index resident in
+HBM, from the call into the library until every record is in host memory. The bench
+times K steps of the batched form (the reported value) and K steps of the plain form,
+checks that both forms give identical results, times the reference's CPU search
+(oracle/_ref, built from the reference's C sources) on a bounded sample of the same
+reads and checks the GPU records against it, and reports the dominant kernel's
+achieved algorithmic bytes/s against MI355X's HBM peak and the measured
+random-access (pointer-chase) ceiling.
+
+Prints one JSON line on stdout; progress goes to stderr.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import re
+import shutil
+import struct
+import subprocess
+import sys
+import tempfile
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "reads/s aligned, bit-exact .sai, 1 GPU, batched vs plain; p50/p99 request ms"
+HBM_PEAK_GBS = 8000.0
+
+
+def log(msg: str) -> None:
+    sys.stderr.write(f"[bench {time.strftime('%H:%M:%S')}] {msg}\n")
+    sys.stderr.flush()
+
+
+def parse_sai(path: str, n: int):
+    """(n_aln[n], list of record arrays) from a .sai file."""
+    raw = open(path, "rb").read()
+    off = 4 + 64
+    n_aln = np.zeros(n, np.int32)
+    recs = []
+    for i in range(n):
+        (na,) = struct.unpack_from("<i", raw, off)
+        off += 4
+        n_aln[i] = na
+        recs.append(np.frombuffer(raw, dtype=np.uint64, count=3 * na, offset=off).reshape(na, 3))
+        off += 24 * na
+    return n_aln, recs
+
+
+def cpu_baseline(w, res, sample: int, threads: int, workdir: str) -> dict:
+    """The reference's CPU search on the first `sample` reads, and its agreement with the GPU."""
+    ref_bin = os.path.join(ROOT, "oracle", "_ref", "galn_ref")
+    fq = w.fastq(os.path.join(workdir, "sample.fq"), 0, sample)
+    sai = os.path.join(workdir, "sample.sai")
+    if os.path.exists(ref_bin):
+        kind = "reference"
+        cmd = [ref_bin, *w.opts, "--il", "8", "--threads", str(threads), "-f", sai, w.prefix, fq]
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode not in (0, 4):
+            raise RuntimeError(f"reference CPU search failed: {r.stderr}")
+        m = re.search(r"(\d+) reads searched in ([0-9.]+) s", r.stderr)
+        secs = float(m.group(2))
+        n_aln, recs = parse_sai(sai, sample)
+    else:
+        # the oracle's restatement, when the reference build is absent
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle as orc
+        kind = "port"
+        o = orc.parse_aln_opts(w.opts)
+        q = w.query
+        t = time.time()
+        bid = orc.batch_ids(sample)
+        bopts, nst, maxd = orc.batch_options(o, w.lens[:sample].astype(np.int32), bid)
+        rr = orc.search(w.words, w.meta, q, w.qoff[:sample], w.lens[:sample].astype(np.int32), maxd, bid, bopts,
+                        nst, o["seed_len"], threads)
+        secs = time.time() - t
+        n_aln = rr.n_aln
+        recs = [rr.records(i) for i in range(sample)]
+    agree = bool((np.where(n_aln > 0, n_aln, 0) == np.where(res.n_aln[:sample] > 0, res.n_aln[:sample], 0)).all())
+    if agree:
+        for i in range(sample):
+            if n_aln[i] > 0 and not np.array_equal(recs[i], res.records(i)):
+                agree = False
+                break
+    return {"value": sample / secs, "unit": "reads/s", "cores": threads, "kind": kind, "seconds": secs,
+            "sample": f"first {sample} reads of the same workload, {threads} host threads, interleaved search",
+            "records_equal_gpu": agree}
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--reads", type=int, default=10_000_000)
+    ap.add_argument("--ref-bp", type=int, default=100_000_000)
+    ap.add_argument("--cpu-sample", type=int, default=0, help="reads for the CPU baseline (0: ~15 s of CPU work)")
+    ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--workdir", default="")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        sys.stderr.write("bench.py: this build measures one GPU only (the search is per-GPU; see DESIGN.md "
+                         "for how reads would shard); run with --gpus 1\n")
+        return 2
+
+    import torch
+
+    from galn_amd import native as nv
+    from galn_amd import workloads
+
+    if not torch.cuda.is_available():
+        sys.stderr.write("bench.py: no GPU visible\n")
+        return 3
+    workdir = args.workdir or tempfile.mkdtemp(prefix="galn_bench_")
+    os.makedirs(workdir, exist_ok=True)
+    threads = args.cpu_threads or min(os.cpu_count() or 1, 16)
+
+    log(f"building workload: {args.ref_bp} bp reference, {args.reads} reads (index on the GPU)")
+    t = time.time()
+    w = workloads.adna_mixed(workdir, ref_bp=args.ref_bp, n_reads=args.reads, device="cuda", workers=threads)
+    log(f"workload ready in {time.time() - t:.1f} s ({w.timings}); n={w.n}, index {w.words.nbytes / 1e6:.1f} MB")
+    torch.cuda.synchronize()
+    torch.cuda.empty_cache()
+
+    # random-access ceiling at the index's working-set size
+    ws = int(w.words.nbytes)
+    best = 0.0
+    for waves in (8, 16, 32):
+        lps, _ = nv.gather_ceiling(max(ws, 1 << 20), 64, waves, 1000)
+        best = max(best, lps)
+    ceiling_gbs = best * 64 / 1e9
+    log(f"pointer-chase ceiling at {ws / 1e6:.1f} MB: {ceiling_gbs:.1f} GB/s of 64-byte lines")
+
+    index = nv.Index(w.words, w.meta, 0)
+    rs = w.readset(index)
+    log("warm-up")
+    for _ in range(max(args.warmup, 0)):
+        rs.search(nv.FORM_BATCHED)
+        rs.results(want_pops=False)
+    if not args.no_plain:
+        rs.search(nv.FORM_PLAIN)
+        res_plain = rs.results(want_pops=False)
+
+    def timed(form: int, steps: int):
+        times, launches = [], []
+        res = None
+        for s in range(steps):
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            rs.search(form)
+            res = rs.results(want_pops=False, want_bytes=(s == steps - 1))
+            torch.cuda.synchronize()
+            times.append(time.perf_counter() - t0)
+            launches.append(rs.launches())
+            log(f"  form {form} step {s}: {times[-1] * 1e3:.1f} ms, {w.n / times[-1]:.3e} reads/s")
+        return times, launches, res
+
+    log(f"timing batched form, {args.steps} steps")
+    tb, lb, res_b = timed(nv.FORM_BATCHED, args.steps)
+    if not args.no_plain:
+        log(f"timing plain form, {args.steps} steps")
+        tp, lp, res_p = timed(nv.FORM_PLAIN, args.steps)
+        same = (np.array_equal(res_p.n_aln, res_b.n_aln) and np.array_equal(res_p.rec, res_b.rec)
+                and np.array_equal(res_plain.n_aln, res_b.n_aln))
+    else:
+        tp, lp, same = [], [], None
+    log(f"plain == batched: {same}")
+
+    ms_b = 1e3 * sum(tb) / len(tb)
+    value = w.n / (ms_b / 1e3)
+    # dominant kernel: the batched form's first-tier persistent launch
+    first = [l for l in lb[-1] if l["tier"] == 0]
+    k_ms = sum(l["ms"] for l in first) / len(first)
+    k_bytes = sum(l["algo_bytes"] for l in first) / len(first)
+    achieved = k_bytes / (k_ms * 1e-3) / 1e9
+    total_algo = int(res_b.occ_bytes.sum())
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "reads/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_b,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "int64",
+        "data": "synthetic (seeded generator; index built on the GPU from the generated reference)",
+        "config": {"workload": "batched aDNA mixed length: 100 Mbp reference (41% GC, 10% duplicated segments, 1% N), "
+                               "10M reads 30-150 bp (median 45), aln -l 1024 -n 0.01 -o 2",
+                   "ref_bp": args.ref_bp, "reads": w.n, "aln_opts": " ".join(w.opts), "form": "batched",
+                   "index_bytes": ws},
+        "plain": {"value": w.n / (sum(tp) / len(tp)) if tp else None,
+                  "ms_per_step": 1e3 * sum(tp) / len(tp) if tp else None},
+        "batched_over_plain": (sum(tp) / len(tp)) / (sum(tb) / len(tb)) if tp else None,
+        "parity": {"plain_eq_batched": same},
+        "kernel": {"name": "k_search<persistent>", "first_tier_ms": k_ms,
+                   "all_launches_ms": sum(l["ms"] for l in lb[-1]), "launches": len(lb[-1]),
+                   "algo_bytes_first_tier": k_bytes, "algo_bytes_total": total_algo},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None},
+        "ceiling": {"kind": "pointer chase, 64-byte random lines", "ws_bytes": ws, "GB/s": ceiling_gbs,
+                    "frac": achieved / ceiling_gbs if ceiling_gbs else None},
+    }
+    if not args.no_cpu:
+        sample = args.cpu_sample
+        if sample <= 0:
+            # calibrate: ~15 s of CPU work
+            cal = min(w.n, 20000)
+            c0 = cpu_baseline(w, res_b, cal, threads, workdir)
+            sample = int(min(w.n, max(cal, 15.0 * c0["value"])))
+        log(f"CPU baseline on {sample} reads, {threads} threads")
+        cb = cpu_baseline(w, res_b, sample, threads, workdir)
+        out["cpu_baseline"] = cb
+        out["parity"]["cpu_sample_eq_gpu"] = cb["records_equal_gpu"]
+        out["parity"]["cpu_sample_reads"] = sample
+        log(f"CPU baseline: {cb['value']:.3e} reads/s ({cb['kind']}), records equal: {cb['records_equal_gpu']}")
+    if not args.workdir:
+        shutil.rmtree(workdir, ignore_errors=True)
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -45,8 +45,8 @@ def build_hip() -> str:
     os.makedirs(LIB, exist_ok=True)
     out = os.path.join(LIB, "libgaln_hip.so")
     tmp = out + ".tmp"
-    _run([HIPCC, *HIP_FLAGS, "-shared", f"-I{INC}", "-o", tmp, os.path.join(CSRC, "galn_search.hip")],
-         quiet_feature_warning=True)
+    _run([HIPCC, *HIP_FLAGS, "-shared", f"-I{INC}", "-o", tmp, os.path.join(CSRC, "galn_search.hip"),
+          os.path.join(CSRC, "galn_membench.hip")], quiet_feature_warning=True)
     os.replace(tmp, out)
     return out
 

@@ -9,9 +9,13 @@
 //
 // Per-lane search state (reference src/bwtgap_hd.h:338-473):
 //   * the priority stack: one LIFO per score bucket, built from 16-entry
-//     chunks of 16-byte packed entries in the lane's global arena; the top
-//     of each bucket (chunk id, fill) lives in LDS, the non-empty-bucket bitmask
-//     in registers, so a pop is one dependent global load and a push is none;
+//     chunks of 16-byte packed entries; the top of each bucket (chunk id,
+//     fill) lives in LDS, the non-empty-bucket bitmask in registers, so a pop
+//     is one dependent global load and a push is none. Chunks come from a
+//     device-wide pool in granules of 64 chunks: a lane takes granules as its
+//     stack grows (lock-free tagged free stack, then a bump counter), keeps a
+//     few for its next read and hands the rest back, so a read's stack grows to
+//     its true peak without re-runs and the pool holds only what is live;
 //   * the exact child of an expansion (pushed last, same score, so always the
 //     next pop) stays in registers instead of going through the arena;
 //   * width[] / seed_width[] as packed u64 (w:48 | bid:16) in the lane's arena.
@@ -22,9 +26,9 @@
 //   plain    one launch per 0x40000 reads (one lane per read, input order);
 //   batched  one persistent launch over the whole read list; lanes claim the
 //            next read from a device counter as soon as theirs is done.
-// Reads that outgrow a lane's first-tier arena (stack or record space) are
-// re-run from scratch in larger tiers on the device, so results never depend
-// on the capacities chosen (the reference's contract, src/bwtgap_hd.h:10-22).
+// Reads that outgrow their record slots, or find the pool exhausted, are re-run
+// from scratch in a later launch with more room, so results never depend on
+// the capacities chosen (the reference's contract, src/bwtgap_hd.h:10-22).
 
 #include <hip/hip_runtime.h>
 
@@ -42,6 +46,8 @@
 namespace {
 
 constexpr int CH = 16;                       // entries per stack chunk
+constexpr int GRAN = 64;                     // chunks per pool granule
+constexpr uint32_t MAX_POOL_CHUNKS = 1u << 28;  // 28-bit chunk ids in the bucket tops
 constexpr int MAXB = 256;                    // score buckets (bitmask width)
 constexpr int MAXLEN = 1023;                 // read length limit (10-bit entry fields)
 constexpr int MAXDIFF = 126;                 // 7-bit n_mm / n_gapo fields (n_mm may reach max_diff + 1)
@@ -55,6 +61,7 @@ constexpr int ERR_SCORE = -3;   // final, as the reference
 constexpr int ERR_UNSUP = -5;
 constexpr int ERR_OUT = -8;     // output arena full: re-run after it grows
 constexpr int ERR_INTERNAL = -9; // stack bookkeeping inconsistent (final)
+constexpr int ERR_HEAVY = -10;   // pop budget spent: the read moves to the wave-per-read engine
 
 constexpr int MODE_GAPE = 0x01, MODE_LOGGAP = 0x04, MODE_NONSTOP = 0x10;
 constexpr int ST_M = 0, ST_I = 1, ST_D = 2;
@@ -66,6 +73,11 @@ struct DMeta {
     uint64_t primary;
     uint64_t L2[5];
     uint64_t seq_len;
+};
+
+// packed stack entry (layout below)
+struct __align__(16) Ent {
+    uint64_t w0, w1;
 };
 
 struct KArgs {
@@ -86,24 +98,37 @@ struct KArgs {
     int32_t n_list;
     int32_t *qnext;          // persistent claims
     int32_t n_slots;
-    // per-lane arena
+    // per-lane region: widths, seed widths, records, owned granules
     uint8_t *scratch;
     uint64_t slot_bytes;
-    uint32_t sw_off, rec_off, link_off, chunk_off;
-    int32_t n_chunks;
+    uint32_t sw_off, rec_off, own_off;
+    int32_t own_cap;         // granules a lane may hold (enough for max_entries)
+    int32_t keep;            // granules a lane keeps between reads
     int32_t rec_cap;
+    // stack pool
+    Ent *pool_chk;           // n_gran * GRAN * CH entries
+    uint32_t *pool_link;     // per chunk: previous chunk of its bucket, or next free chunk of its lane
+    uint32_t *gnext;         // per granule: next granule in the free stack
+    unsigned long long *ghead;  // free stack head: tag << 32 | granule (0xffffffff = empty)
+    uint32_t *gbump;         // granules never handed out start here
+    uint32_t n_gran;
     // results (indexed by read id)
     int32_t *out_n;
     uint64_t *out_off;
     uint64_t *out_pops;
     uint64_t *out_trace;
+    uint64_t *out_bytes;     // algorithmic index bytes per read
     galn_aln_rec_t *out_rec;
     uint64_t out_cap;
     unsigned long long *out_top;
     int32_t *fail_list;
     int32_t *fail_n;
     int32_t *fail_kind;      // [0] output full, [1] record slots, [2] stack arena
+    unsigned long long *algo_bytes;  // this launch's algorithmic index bytes (final results only)
     int32_t fuse;
+    uint64_t pop_budget;     // > 0: reads popping more entries go to heavy_list
+    int32_t *heavy_list;
+    int32_t *heavy_n;
 };
 
 // ---------------------------------------------------------------------------
@@ -144,8 +169,8 @@ __device__ __forceinline__ void block_occ4(const uint4 (&b)[4], int r, uint64_t 
 
 // Occ of all four bases at positions a (may be -1) and b >= 0: bwt_2occ4, src/bwtgap_hd.h:158-192.
 // One 64-byte block when both fall in it, else two independent loads issued together.
-__device__ __forceinline__ void lf_pair(const uint32_t *__restrict__ bwt, const DMeta &m, uint64_t a, uint64_t b,
-                                        uint64_t ca[4], uint64_t cb[4])
+__device__ __forceinline__ uint32_t lf_pair(const uint32_t *__restrict__ bwt, const DMeta &m, uint64_t a, uint64_t b,
+                                            uint64_t ca[4], uint64_t cb[4])
 {
     const uint64_t bp = b - (b >= m.primary);
     const uint64_t bb = bp >> 7;
@@ -164,6 +189,7 @@ __device__ __forceinline__ void lf_pair(const uint32_t *__restrict__ bwt, const 
     } else {
         block_occ4(A, (int)(ap & 127), ca);
     }
+    return (aneg || same) ? 64u : 128u;
 }
 
 // ---------------------------------------------------------------------------
@@ -173,9 +199,6 @@ __device__ __forceinline__ void lf_pair(const uint32_t *__restrict__ bwt, const 
 // n_del = n_gapo + n_gape - n_ins (every gap open or extension adds one
 // inserted or one deleted base, src/bwtgap_hd.h:430-454).
 
-struct __align__(16) Ent {
-    uint64_t w0, w1;
-};
 
 __device__ __forceinline__ Ent mk_ent(uint64_t k, uint64_t l, int i, int ldp, int mm, int go, int ge, int ins, int st)
 {
@@ -204,9 +227,45 @@ __device__ __forceinline__ uint64_t wpack(uint64_t w, int bid) { return w | ((ui
 
 enum : int { PH_CW = 0, PH_POPPED = 1, PH_EXACT = 2 };
 
-template <bool TRACE, bool PERSIST, typename TopT>
+// Free granules: a lock-free stack whose head carries a tag bumped on every change
+// (no ABA), links read and written at device scope; then never-used granules.
+__device__ __forceinline__ uint32_t gran_pop(unsigned long long *ghead, uint32_t *gnext, uint32_t *gbump, uint32_t n_gran)
+{
+    for (;;) {
+        unsigned long long old = __hip_atomic_load(ghead, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+        const uint32_t g = (uint32_t)old;
+        if (g == NIL) break;
+        const uint32_t nxt = __hip_atomic_load(gnext + g, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        const unsigned long long nw = ((old >> 32) + 1) << 32 | nxt;
+        if (__hip_atomic_compare_exchange_strong(ghead, &old, nw, __ATOMIC_ACQ_REL, __ATOMIC_RELAXED,
+                                                 __HIP_MEMORY_SCOPE_AGENT))
+            return g;
+    }
+    const uint32_t b = atomicAdd(gbump, 1u);
+    return b < n_gran ? b : NIL;
+}
+
+// push the chain own[from..to) (linked through gnext) in one exchange
+__device__ __forceinline__ void gran_push(unsigned long long *ghead, uint32_t *gnext, const uint32_t *own, int from,
+                                          int to)
+{
+    for (int j = from; j + 1 < to; ++j)
+        __hip_atomic_store(gnext + own[j], own[j + 1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const uint32_t first = own[from], last = own[to - 1];
+    unsigned long long old = __hip_atomic_load(ghead, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    for (;;) {
+        __hip_atomic_store(gnext + last, (uint32_t)old, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        const unsigned long long nw = ((old >> 32) + 1) << 32 | first;
+        if (__hip_atomic_compare_exchange_strong(ghead, &old, nw, __ATOMIC_RELEASE, __ATOMIC_RELAXED,
+                                                 __HIP_MEMORY_SCOPE_AGENT))
+            return;
+    }
+}
+
+template <bool TRACE, bool PERSIST>
 __global__ void __launch_bounds__(128) k_search(KArgs a)
 {
+    using TopT = uint32_t;  // chunk id << 4 | fill - 1
     extern __shared__ __align__(16) uint8_t lds_raw[];
     const int slot = blockIdx.x * blockDim.x + threadIdx.x;
     if (slot >= a.n_slots) return;
@@ -217,8 +276,9 @@ __global__ void __launch_bounds__(128) k_search(KArgs a)
     uint64_t *const W = reinterpret_cast<uint64_t *>(base);
     uint64_t *const SW = reinterpret_cast<uint64_t *>(base + a.sw_off);
     galn_aln_rec_t *const REC = reinterpret_cast<galn_aln_rec_t *>(base + a.rec_off);
-    uint32_t *const LINK = reinterpret_cast<uint32_t *>(base + a.link_off);
-    Ent *const CHK = reinterpret_cast<Ent *>(base + a.chunk_off);
+    uint32_t *const OWN = reinterpret_cast<uint32_t *>(base + a.own_off);
+    uint32_t *const LINK = a.pool_link;
+    Ent *const CHK = a.pool_chk;
     const DMeta &M = a.meta;
     const uint64_t seq_len = M.seq_len;
 
@@ -241,19 +301,22 @@ __global__ void __launch_bounds__(128) k_search(KArgs a)
     bool has_cur = false;
     Ent cur{0, 0}, e{0, 0};
     int cur_score = 0, e_score = 0, m = 0, m_seed = 0;
-    uint32_t spare = NIL, free_head = NIL, bump = 0;
-    uint64_t pops = 0, trace = TR_INIT;
+    uint32_t spare = NIL, free_head = NIL;
+    int n_own = 0, own_next = 0, gleft = 0;  // granules held, next one to use, chunks left in it
+    uint32_t gbase = 0;
+    uint64_t pops = 0, trace = TR_INIT, obytes = 0;
+    uint32_t lf_bytes = 0;
     // LF request
     bool need_lf = false;
     uint64_t qa = 0, qb = 0;
     bool need_claim = true, claimed = false;
 
     // ---- helpers (all lane-local) ----
-    auto bm_test = [&](int s) -> bool {
+    auto bm_test = [&](int s) __attribute__((always_inline)) -> bool {
         const uint64_t w = s < 64 ? bm0 : s < 128 ? bm1 : s < 192 ? bm2 : bm3;
         return (w >> (s & 63)) & 1ull;
     };
-    auto bm_set = [&](int s) {
+    auto bm_set = [&](int s) __attribute__((always_inline)) {
         const uint64_t bit = 1ull << (s & 63);
         const int w = s >> 6;
         bm0 |= w == 0 ? bit : 0ull;
@@ -261,7 +324,7 @@ __global__ void __launch_bounds__(128) k_search(KArgs a)
         bm2 |= w == 2 ? bit : 0ull;
         bm3 |= w == 3 ? bit : 0ull;
     };
-    auto bm_clear = [&](int s) {
+    auto bm_clear = [&](int s) __attribute__((always_inline)) {
         const uint64_t bit = ~(1ull << (s & 63));
         const int w = s >> 6;
         bm0 &= w == 0 ? bit : ~0ull;
@@ -269,11 +332,11 @@ __global__ void __launch_bounds__(128) k_search(KArgs a)
         bm2 &= w == 2 ? bit : ~0ull;
         bm3 &= w == 3 ? bit : ~0ull;
     };
-    auto bm_low = [&]() -> int {
+    auto bm_low = [&]() __attribute__((always_inline)) -> int {
         return bm0 ? __builtin_ctzll(bm0)
                    : bm1 ? 64 + __builtin_ctzll(bm1) : bm2 ? 128 + __builtin_ctzll(bm2) : 192 + __builtin_ctzll(bm3);
     };
-    auto chunk_alloc = [&]() -> uint32_t {
+    auto chunk_alloc = [&]() __attribute__((always_inline)) -> uint32_t {
         uint32_t c;
         if (spare != NIL) {
             c = spare;
@@ -281,14 +344,27 @@ __global__ void __launch_bounds__(128) k_search(KArgs a)
         } else if (free_head != NIL) {
             c = free_head;
             free_head = LINK[c];
-        } else if (bump < (uint32_t)a.n_chunks) {
-            c = bump++;
+        } else if (gleft > 0) {
+            c = gbase + (uint32_t)(GRAN - gleft);
+            --gleft;
         } else {
-            c = NIL;
+            uint32_t g;
+            if (own_next < n_own) {
+                g = OWN[own_next];
+            } else {
+                if (n_own >= a.own_cap) return NIL;
+                g = gran_pop(a.ghead, a.gnext, a.gbump, a.n_gran);
+                if (g == NIL) return NIL;
+                OWN[n_own++] = g;
+            }
+            ++own_next;
+            gbase = g * GRAN;
+            gleft = GRAN - 1;
+            c = gbase;
         }
         return c;
     };
-    auto chunk_free = [&](uint32_t c) {
+    auto chunk_free = [&](uint32_t c) __attribute__((always_inline)) {
         if (spare == NIL) {
             spare = c;
         } else {
@@ -297,7 +373,7 @@ __global__ void __launch_bounds__(128) k_search(KArgs a)
         }
     };
     // gap_push into the arena: src/bwtgap_hd.h:264-286
-    auto push_mem = [&](int s, const Ent &en) -> int {
+    auto push_mem = [&](int s, const Ent &en) __attribute__((always_inline)) -> int {
         if (s < 0 || s >= n_stacks) return ERR_SCORE;
         TopT *tp = tops + s * tstride;
         uint32_t chunk, idx;
@@ -326,7 +402,7 @@ __global__ void __launch_bounds__(128) k_search(KArgs a)
         return 0;
     };
     // a child: the exact one (same score, pushed last) is kept in registers
-    auto push = [&](int s, const Ent &en, bool exact) -> int {
+    auto push = [&](int s, const Ent &en, bool exact) __attribute__((always_inline)) -> int {
         if (exact && a.fuse) {
             cur = en;
             cur_score = s;
@@ -336,11 +412,21 @@ __global__ void __launch_bounds__(128) k_search(KArgs a)
         }
         return push_mem(s, en);
     };
-    auto score_of = [&](int mm, int go, int ge) -> int { return mm * s_mm + go * s_gapo + ge * s_gape; };
+    auto score_of = [&](int mm, int go, int ge) __attribute__((always_inline)) -> int { return mm * s_mm + go * s_gapo + ge * s_gape; };
 
     // result of the current read; the lane then claims the next one
-    auto finish = [&](int code) {
+    auto finish = [&](int code) __attribute__((always_inline)) {
         int n = code;
+        if (n == ERR_HEAVY) {
+            a.heavy_list[atomicAdd(a.heavy_n, 1)] = rid;
+            if (n_own > a.keep) {
+                gran_push(a.ghead, a.gnext, OWN, a.keep, n_own);
+                n_own = a.keep;
+            }
+            need_lf = false;
+            need_claim = true;
+            return;
+        }
         if (n > 0) {
             const unsigned long long b0 = atomicAdd(a.out_top, (unsigned long long)n);
             if (b0 + (unsigned long long)n > a.out_cap) {
@@ -358,12 +444,18 @@ __global__ void __launch_bounds__(128) k_search(KArgs a)
         a.out_n[rid] = n;
         a.out_pops[rid] = pops;
         if (TRACE) a.out_trace[rid] = trace;
+        if (a.out_bytes) a.out_bytes[rid] = obytes;
+        if (n != ERR_STACK && n != ERR_ALN && n != ERR_OUT) atomicAdd(a.algo_bytes, (unsigned long long)obytes);
+        if (n_own > a.keep) {
+            gran_push(a.ghead, a.gnext, OWN, a.keep, n_own);
+            n_own = a.keep;
+        }
         need_lf = false;
         need_claim = true;
     };
 
     // gap_shadow: src/bwtgap_hd.h:309-320
-    auto shadow = [&](int x, int ldp) {
+    auto shadow = [&](int x, int ldp) __attribute__((always_inline)) {
         const uint64_t ux = (uint64_t)(int64_t)x;
         int j = 0;
         for (int i = 0; i < ldp; ++i) {
@@ -383,7 +475,7 @@ __global__ void __launch_bounds__(128) k_search(KArgs a)
     };
 
     // a hit (src/bwtgap_hd.h:385-409); returns true when the search stops here
-    auto hit = [&](uint64_t k, uint64_t l) -> bool {
+    auto hit = [&](uint64_t k, uint64_t l) __attribute__((always_inline)) -> bool {
         const int mm = e_mm(e), go = e_go(e), ge = e_ge(e);
         const int score = e_score;
         bool do_add = true;
@@ -427,7 +519,7 @@ __global__ void __launch_bounds__(128) k_search(KArgs a)
 
     // the top of bwt_match_gap's loop: pop entries until one needs an LF lookup
     // or the search ends (src/bwtgap_hd.h:357-383)
-    auto advance = [&]() {
+    auto advance = [&]() __attribute__((always_inline)) {
         for (;;) {
             if (n_entries == 0 || n_entries > max_entries) {
                 finish(n_aln);
@@ -461,6 +553,10 @@ __global__ void __launch_bounds__(128) k_search(KArgs a)
             }
             --n_entries;
             ++pops;
+            if (a.pop_budget && pops > a.pop_budget) {
+                finish(ERR_HEAVY);
+                return;
+            }
             const int i0 = e_i(e), mm = e_mm(e), go = e_go(e), ge = e_ge(e), st = e_st(e);
             if (TRACE) {
                 const int ins = e_ins(e), del = go + ge - ins;
@@ -497,7 +593,7 @@ __global__ void __launch_bounds__(128) k_search(KArgs a)
         }
     };
 
-    auto start_search = [&]() {
+    auto start_search = [&]() __attribute__((always_inline)) {
         int nN = 0;
         for (int j = 0; j < len; ++j) nN += q[j] > 3;
         best_score = score_of(md_read + 1, max_gapo + 1, max_gape + 1);
@@ -510,7 +606,8 @@ __global__ void __launch_bounds__(128) k_search(KArgs a)
         bm0 = bm1 = bm2 = bm3 = 0;
         n_entries = 0;
         spare = free_head = NIL;
-        bump = 0;
+        own_next = 0;
+        gleft = 0;
         has_cur = false;
         if (nN > md_read) {
             finish(0);
@@ -526,7 +623,7 @@ __global__ void __launch_bounds__(128) k_search(KArgs a)
     };
 
     // widths: step to the next position that needs an LF lookup (N restarts inline)
-    auto prep_width = [&]() {
+    auto prep_width = [&]() __attribute__((always_inline)) {
         for (;;) {
             if (cj == plen) {
                 wout[plen] = wpack(0, ++cbid);
@@ -562,7 +659,7 @@ __global__ void __launch_bounds__(128) k_search(KArgs a)
         }
     };
 
-    auto init_read = [&](int pos) {
+    auto init_read = [&](int pos) __attribute__((always_inline)) {
         rid = a.list ? a.list[pos] : a.list_base + pos;
         len = a.qlen[rid];
         q = a.query + a.qoff[rid];
@@ -584,6 +681,7 @@ __global__ void __launch_bounds__(128) k_search(KArgs a)
         seeding = seed_len < len;
         pops = 0;
         trace = TR_INIT;
+        obytes = 0;
         if (len < 1 || len > MAXLEN || n_stacks > MAXB || md_read > MAXDIFF || max_gapo > MAXDIFF ||
             max_gapo + max_gape > 255 || max_gape < 0) {
             finish(ERR_UNSUP);
@@ -634,10 +732,11 @@ __global__ void __launch_bounds__(128) k_search(KArgs a)
             }
         }
         uint64_t ca[4], cb[4];
-        if (need_lf) lf_pair(a.bwt, M, qa, qb, ca, cb);
+        if (need_lf) lf_bytes = lf_pair(a.bwt, M, qa, qb, ca, cb);
         need_lf = false;
 
         if (phase == PH_CW) {
+            obytes += lf_bytes;
             const int c = cbase;
             uint64_t k = M.L2[c] + ca[c] + 1, l = M.L2[c] + cb[c];
             if (k > l) {
@@ -650,74 +749,65 @@ __global__ void __launch_bounds__(128) k_search(KArgs a)
             cl = l;
             ++cj;
             prep_width();
-        } else if (phase == PH_EXACT) {
-            // bwt_match_exact_alt step: src/bwtgap_hd.h:201-208
-            const int c = cbase;
-            const uint64_t k = M.L2[c] + ca[c] + 1, l = M.L2[c] + cb[c];
-            if (k > l) {
-                advance();
-                continue;
-            }
-            --cj;
-            if (cj < 0) {
-                if ((int)(uint32_t)(l - k + 1) != 0) {
-                    if (hit(k, l)) continue;
+        } else {
+            // exact tails (bwt_match_exact_alt, src/bwtgap_hd.h:201-208) and popped entries
+            // (src/bwtgap_hd.h:375-470) end in one of: the next LF lookup, a hit, or the next pop
+            int act = 0;  // 0: LF lookup set up, 1: pop next, 2: hit at (hk, hl) then pop next
+            uint64_t hk = 0, hl = 0;
+            bool expand = false;
+            int c = cbase, j_next = cj - 1;
+            if (phase == PH_POPPED) {
+                const int i0 = e_i(e);
+                const int st = e_st(e), ge = e_ge(e);
+                if (m < wbid(wB)) {
+                    act = 1;
+                } else if (m == 0 && (st == ST_M || (mode & MODE_GAPE) || ge == max_gape)) {
+                    // exact tail: its first step is this lookup
+                    c = qbase;
+                    j_next = i0 - 2;
+                    phase = PH_EXACT;
+                    if (qbase > 3) act = 1;
+                } else {
+                    expand = true;
                 }
-                advance();
-                continue;
             }
-            const int c2 = q[cj];
-            if (c2 > 3) {
-                advance();
-                continue;
-            }
-            cbase = c2;
-            ck = k;
-            cl = l;
-            qa = k - 1;
-            qb = l;
-            need_lf = true;
-        } else {  // PH_POPPED: src/bwtgap_hd.h:375-470
-            const int i0 = e_i(e);
-            if (m < wbid(wB)) {
-                advance();
-                continue;
-            }
-            const int st = e_st(e), ge = e_ge(e);
-            if (m == 0 && (st == ST_M || (mode & MODE_GAPE) || ge == max_gape)) {
-                // exact tail, first step from this lookup
-                if (qbase > 3) {
-                    advance();
-                    continue;
-                }
-                const uint64_t k = M.L2[qbase] + ca[qbase] + 1, l = M.L2[qbase] + cb[qbase];
+            if (phase == PH_EXACT && act == 0) {
+                obytes += lf_bytes;
+                const uint64_t k = M.L2[c] + ca[c] + 1, l = M.L2[c] + cb[c];
                 if (k > l) {
-                    advance();
-                    continue;
-                }
-                cj = i0 - 2;
-                if (cj < 0) {
-                    if ((int)(uint32_t)(l - k + 1) != 0) {
-                        if (hit(k, l)) continue;
+                    act = 1;
+                } else if (j_next < 0) {
+                    act = (int)(uint32_t)(l - k + 1) != 0 ? 2 : 1;
+                    hk = k;
+                    hl = l;
+                } else {
+                    const int c2 = q[j_next];
+                    if (c2 > 3) {
+                        act = 1;
+                    } else {
+                        cj = j_next;
+                        cbase = c2;
+                        ck = k;
+                        cl = l;
+                        qa = k - 1;
+                        qb = l;
+                        need_lf = true;
                     }
-                    advance();
-                    continue;
                 }
-                const int c2 = q[cj];
-                if (c2 > 3) {
-                    advance();
-                    continue;
-                }
-                cbase = c2;
-                ck = k;
-                cl = l;
-                qa = k - 1;
-                qb = l;
-                need_lf = true;
-                phase = PH_EXACT;
+            }
+            if (act == 2) {
+                if (hit(hk, hl)) continue;
+                act = 1;
+            }
+            if (act == 1) {
+                advance();
                 continue;
             }
+            if (!expand) continue;
+            const int i0 = e_i(e);
+            const int st = e_st(e), ge = e_ge(e);
             // expansion
+            obytes += lf_bytes;
             const int i = i0 - 1;
             const int mm = e_mm(e), go = e_go(e), ins = e_ins(e), del = go + ge - ins;
             const uint64_t k = e_k(e), l = e_l(e), occ = l - k + 1;
@@ -791,6 +881,754 @@ __global__ void __launch_bounds__(128) k_search(KArgs a)
 }
 
 // ---------------------------------------------------------------------------
+// Wave-per-read engine for heavy reads.
+//
+// Between two hits the search's state (max_diff, widths, best score) is fixed,
+// and with positive penalties every non-exact child lands in a higher bucket
+// than its parent. So the entries of the lowest bucket b each start an
+// independent "chain" -- the entry, then its exact child, then that child's
+// exact child, ... (the sequential search pops exactly this chain next, LIFO) --
+// whose pushes all go to buckets above b. A wave takes the top K <= 64 entries
+// of bucket b, runs the K chains in parallel (one lane each, pushes logged),
+// then commits them in the sequential order: pops, pushes (placed by a
+// per-bucket scan so every bucket keeps its LIFO order), removal of the K
+// entries, and the first hit, which changes the state and ends the commit.
+// Chains after that hit are discarded and re-run in the next round. The
+// result, the pop count and the trace are those of the sequential search.
+//
+// Buckets are LIFO lists of segments (a 16-byte header naming the segment
+// below, then the entries) in granules the wave takes from the stack pool.
+
+constexpr int HW = 64;         // lanes, one wave per workgroup
+constexpr int H_STEPS = 32;    // chain steps per lane per round
+constexpr int H_PUSH = 128;    // logged pushes per lane per round
+constexpr int H_CAP = 4096;    // pushes committed per round
+constexpr int H_PIECES = MAXB + 16;
+constexpr uint32_t GENT = (uint32_t)GRAN * CH;  // entries per granule
+
+enum : int { K_END = 0, K_HIT = 1, K_TRUNC = 2, K_ESCORE = 3 };
+
+struct HArgs {
+    const uint32_t *bwt;
+    DMeta meta;
+    const uint8_t *query;
+    const uint64_t *qoff;
+    const int32_t *qlen;
+    const int32_t *maxdiff;
+    const int32_t *batch;
+    const galn_search_opt_t *bopts;
+    const int32_t *nstacks;
+    int32_t seed_len;
+    const int32_t *list;
+    int32_t n_list;
+    int32_t *qnext;
+    int32_t n_slots;
+    uint8_t *scratch;
+    uint64_t slot_bytes;
+    uint32_t rec_off, own_off, plog_off, pscore_off, slog_off, sinfo_off;
+    int32_t own_cap, rec_cap;
+    Ent *pool;
+    uint32_t *gnext;
+    unsigned long long *ghead;
+    uint32_t *gbump;
+    uint32_t n_gran;
+    int32_t *out_n;
+    uint64_t *out_off, *out_pops, *out_trace, *out_bytes;
+    galn_aln_rec_t *out_rec;
+    uint64_t out_cap;
+    unsigned long long *out_top;
+    int32_t *fail_list, *fail_n, *fail_kind;
+    unsigned long long *algo_bytes;
+    int32_t buckets;  // LDS buckets: the largest n_stacks of the launch
+    int32_t wlen;     // LDS widths: longest read + 1
+    int32_t swlen;    // LDS seed widths
+    unsigned long long *stats;  // [8] engine counters (rounds, chains, steps, cycles)
+};
+
+struct HShared {
+    int rid, len, n_stacks, md_read, seeding;
+    int s_mm, s_gapo, s_gape, mode, ies, mdo, max_entries, max_gapo, max_gape, msd, max_top2;
+    int best_score, best_diff, max_diff, best_cnt, n_aln, n_entries;
+    unsigned long long pops, trace, obytes;
+    unsigned long long bm[4];
+    int has_pend, claim, status, result, K, b, J, jcap, jcut, tcut, n_piece, n_own, aleft;
+    uint32_t abase;
+    int nN;
+    uint64_t hk, hl;
+    Ent pend, he;
+    int he_score;
+};
+
+__device__ __forceinline__ int hbm_low(const unsigned long long *bm)
+{
+    return bm[0] ? __builtin_ctzll(bm[0]) : bm[1] ? 64 + __builtin_ctzll(bm[1]) : bm[2] ? 128 + __builtin_ctzll(bm[2])
+                 : bm[3] ? 192 + __builtin_ctzll(bm[3]) : -1;
+}
+
+// bwt_cal_width into LDS by one lane (src/bwtgap_hd.h:216-242); returns the algorithmic bytes
+__device__ uint64_t cal_width_one(const uint32_t *bwt, const DMeta &M, int len, const uint8_t *str, uint64_t *w)
+{
+    uint64_t k = 0, l = M.seq_len, ca[4], cb[4], bytes = 0;
+    int bid = 0;
+    for (int i = 0; i < len; ++i) {
+        const int c0 = str[i];
+        const int c = c0 > 3 ? 4 : 3 - c0;
+        if (c < 4) {
+            bytes += lf_pair(bwt, M, k - 1, l, ca, cb);
+            k = M.L2[c] + ca[c] + 1;
+            l = M.L2[c] + cb[c];
+        }
+        if (k > l || c > 3) {
+            k = 0;
+            l = M.seq_len;
+            ++bid;
+        }
+        w[i] = wpack(l - k + 1, bid);
+    }
+    w[len] = wpack(0, ++bid);
+    return bytes;
+}
+
+template <bool TRACE>
+__global__ void __launch_bounds__(64) k_heavy(HArgs a)
+{
+    extern __shared__ __align__(16) uint8_t hlds[];
+    HShared &F = *reinterpret_cast<HShared *>(hlds);
+    uint8_t *lp = hlds + ((sizeof(HShared) + 15) & ~(size_t)15);
+    const int NB = a.buckets;
+    uint32_t *seg_base = reinterpret_cast<uint32_t *>(lp);
+    uint32_t *seg_cnt = seg_base + NB;
+    uint32_t *Tb = seg_cnt + NB;
+    int32_t *first_piece = reinterpret_cast<int32_t *>(Tb + NB);
+    uint16_t *cnt = reinterpret_cast<uint16_t *>(first_piece + NB);  // cnt[s * 64 + lane]
+    lp = reinterpret_cast<uint8_t *>(cnt + NB * HW);
+    lp = hlds + (((size_t)(lp - hlds) + 15) & ~(size_t)15);
+    uint4 *pieces = reinterpret_cast<uint4 *>(lp);  // {bucket, rank0, len, base}
+    uint64_t *W = reinterpret_cast<uint64_t *>(pieces + H_PIECES);
+    uint64_t *SW = W + a.wlen;
+    int32_t *lsteps = reinterpret_cast<int32_t *>(SW + a.swlen);
+    int32_t *lnp = lsteps + HW, *lkind = lnp + HW, *lcommit = lkind + HW, *lpexcl = lcommit + HW, *lnstart = lpexcl + HW;
+    int32_t *lcut = lnstart + HW;
+
+    const int lane = threadIdx.x;
+    if ((int)blockIdx.x >= a.n_slots) return;
+    uint8_t *const base = a.scratch + (uint64_t)blockIdx.x * a.slot_bytes;
+    galn_aln_rec_t *const REC = reinterpret_cast<galn_aln_rec_t *>(base + a.rec_off);
+    uint32_t *const OWN = reinterpret_cast<uint32_t *>(base + a.own_off);
+    Ent *const PLOG = reinterpret_cast<Ent *>(base + a.plog_off) + lane * H_PUSH;
+    uint8_t *const PSC = base + a.pscore_off + lane * H_PUSH;
+    Ent *const SLOG_ALL = reinterpret_cast<Ent *>(base + a.slog_off);
+    uint32_t *const SINFO_ALL = reinterpret_cast<uint32_t *>(base + a.sinfo_off);
+    Ent *const SLOG = SLOG_ALL + lane * H_STEPS;
+    uint32_t *const SINFO = SINFO_ALL + lane * H_STEPS;
+    Ent *const POOL = a.pool;
+    const DMeta &M = a.meta;
+    const uint64_t seq_len = M.seq_len;
+
+    for (;;) {
+        if (lane == 0) F.claim = atomicAdd(a.qnext, 1);
+        __syncthreads();
+        const int pos = F.claim;
+        if (pos >= a.n_list) break;
+        // ---- read set-up (lane 0), widths (lanes 0 and 1) ----
+        const int rid = a.list[pos];
+        const int len = a.qlen[rid];
+        const uint8_t *q = a.query + a.qoff[rid];
+        if (lane == 0) {
+            const int b0 = a.batch[rid];
+            const galn_search_opt_t o = a.bopts[b0];
+            F.rid = rid;
+            F.len = len;
+            F.s_mm = o.s_mm;
+            F.s_gapo = o.s_gapo;
+            F.s_gape = o.s_gape;
+            F.mode = o.mode;
+            F.ies = o.indel_end_skip;
+            F.mdo = o.max_del_occ;
+            F.max_entries = o.max_entries;
+            F.max_gapo = o.max_gapo;
+            F.max_gape = o.max_gape;
+            F.msd = o.max_seed_diff;
+            F.max_top2 = o.max_top2;
+            F.n_stacks = a.nstacks[b0];
+            F.md_read = a.maxdiff[rid];
+            F.seeding = a.seed_len < len;
+            F.nN = 0;
+            F.obytes = 0;
+            F.n_own = 0;
+            F.aleft = 0;
+            F.abase = 0;
+            F.status = 0;
+        }
+        __syncthreads();
+        const bool unsup = len < 1 || len > MAXLEN || len + 1 > a.wlen || F.n_stacks > NB || F.md_read > MAXDIFF ||
+                           F.max_gapo > MAXDIFF || F.max_gapo + F.max_gape > 255;
+        if (!unsup) {
+            uint64_t wb = 0;
+            if (lane == 0) wb = cal_width_one(a.bwt, M, len, q, W);
+            if (lane == 1 && F.seeding) wb = cal_width_one(a.bwt, M, a.seed_len, q + (len - a.seed_len), SW);
+            int nN = 0;
+            for (int j = lane; j < len; j += HW) nN += q[j] > 3;
+            if (nN) atomicAdd(&F.nN, nN);
+            if (wb) atomicAdd(&F.obytes, (unsigned long long)wb);
+        }
+        __syncthreads();
+        // ---- search set-up ----
+        if (lane == 0) {
+            F.best_score = (F.md_read + 1) * F.s_mm + (F.max_gapo + 1) * F.s_gapo + (F.max_gape + 1) * F.s_gape;
+            F.best_diff = F.md_read + 1;
+            F.max_diff = F.md_read;
+            F.best_cnt = 0;
+            F.n_aln = 0;
+            F.pops = 0;
+            F.trace = TR_INIT;
+            F.bm[0] = F.bm[1] = F.bm[2] = F.bm[3] = 0;
+            F.has_pend = 0;
+            F.n_entries = 0;
+            if (unsup) {
+                F.status = 1;
+                F.result = ERR_UNSUP;
+            } else if (F.nN > F.md_read) {
+                F.status = 1;
+                F.result = 0;
+            } else if (F.n_stacks <= 0) {
+                F.status = 1;
+                F.result = ERR_SCORE;
+            } else {
+                F.has_pend = 1;
+                F.pend = mk_ent(0, seq_len, len, 0, 0, 0, 0, 0, ST_M);
+                F.b = 0;
+                F.n_entries = 1;
+            }
+        }
+        __syncthreads();
+        // ---- rounds ----
+        while (F.status == 0) {
+            if (lane == 0) {
+                // a pending exact child is the top of the lowest bucket (the bucket it came from)
+                const int b = F.has_pend ? F.b : hbm_low(F.bm);
+                if (F.has_pend == 0 && b < 0) {
+                    F.status = 1;
+                    F.result = F.n_aln;
+                } else if (F.n_entries > F.max_entries) {
+                    F.status = 1;
+                    F.result = F.n_aln;
+                } else {
+                    F.b = b;
+                    const bool bit = (F.bm[b >> 6] >> (b & 63)) & 1ull;
+                    const uint32_t avail = (uint32_t)F.has_pend + (bit ? seg_cnt[b] : 0u);
+                    if (!(F.mode & MODE_NONSTOP) && (uint32_t)b > (uint32_t)(F.best_score + F.s_mm)) {
+                        // this pop ends the search (src/bwtgap_hd.h:366)
+                        const Ent e = F.has_pend ? F.pend : POOL[seg_base[b] + seg_cnt[b] - 1];
+                        F.pops += 1;
+                        F.n_entries -= 1;
+                        if (TRACE) {
+                            const int i0 = e_i(e), mm = e_mm(e), go = e_go(e), ge = e_ge(e), st = e_st(e), ins = e_ins(e);
+                            uint64_t t = F.trace;
+                            t = tr_mix(t, (uint64_t)((uint32_t)b << 21 | (uint32_t)i0));
+                            t = tr_mix(t, e_k(e));
+                            t = tr_mix(t, e_l(e));
+                            t = tr_mix(t, (uint64_t)((uint32_t)mm | (uint32_t)go << 8 | (uint32_t)ge << 16 | (uint32_t)st << 24));
+                            t = tr_mix(t, (uint64_t)ins | (uint64_t)(go + ge - ins) << 16);
+                            t = tr_mix(t, (uint64_t)(uint32_t)e_ldp(e));
+                            t = tr_mix(t, (uint64_t)(uint32_t)F.n_entries);
+                            F.trace = t;
+                        }
+                        F.status = 1;
+                        F.result = F.n_aln;
+                    } else {
+                        F.K = (int)min(avail, (uint32_t)HW);
+                    }
+                }
+            }
+            __syncthreads();
+            if (F.status) break;
+            // ---- speculation: lane j runs the chain of the j-th entry from the top of bucket b ----
+            const int b = F.b, K = F.K, hp = F.has_pend;
+            const long long t_spec = wall_clock64();
+            const int s_mm = F.s_mm, s_gapo = F.s_gapo, s_gape = F.s_gape, mode = F.mode, max_diff = F.max_diff;
+            const int n_stacks = F.n_stacks, max_gapo = F.max_gapo, max_gape = F.max_gape, ies = F.ies, mdo = F.mdo;
+            const int msd = F.msd, seed_len = a.seed_len;
+            const bool seeding = F.seeding != 0;
+            int steps = 0, np = 0, kind = K_END;
+            uint64_t hk = 0, hl = 0;
+            Ent nxt{0, 0};
+            if (lane < K) {
+                Ent e = (lane == 0 && hp) ? F.pend : POOL[seg_base[b] + seg_cnt[b] - 1 - (uint32_t)(lane - hp)];
+                for (;;) {
+                    if (steps == H_STEPS || np > H_PUSH - 9) {
+                        kind = K_TRUNC;
+                        nxt = e;
+                        break;
+                    }
+                    SLOG[steps] = e;
+                    uint32_t sp = 0, sbytes = 0;
+                    bool cont = false;
+                    Ent child{0, 0};
+                    const int i0 = e_i(e), mm = e_mm(e), go = e_go(e), ge = e_ge(e), st = e_st(e), ins = e_ins(e);
+                    int m = max_diff - (mm + go);
+                    if (mode & MODE_GAPE) m -= ge;
+                    if (m >= 0) {
+                        int m_seed = 0;
+                        if (seeding) {
+                            m_seed = msd - (mm + go);
+                            if (mode & MODE_GAPE) m_seed -= ge;
+                        }
+                        if (i0 == 0) {
+                            kind = K_HIT;
+                            hk = e_k(e);
+                            hl = e_l(e);
+                        } else {
+                            const uint64_t wB = W[i0 - 1], wA = i0 >= 2 ? W[i0 - 2] : 0;
+                            const int qbase = q[i0 - 1];
+                            uint64_t ca[4], cb[4];
+                            const uint32_t lfb = lf_pair(a.bwt, M, e_k(e) - 1, e_l(e), ca, cb);
+                            if (m < wbid(wB)) {
+                                // pruned
+                            } else if (m == 0 && (st == ST_M || (mode & MODE_GAPE) || ge == max_gape)) {
+                                // exact tail: bwt_match_exact_alt, src/bwtgap_hd.h:195-211
+                                if (qbase <= 3) {
+                                    sbytes += lfb;
+                                    uint64_t k = M.L2[qbase] + ca[qbase] + 1, l = M.L2[qbase] + cb[qbase];
+                                    bool ok = k <= l;
+                                    for (int jn = i0 - 2; ok && jn >= 0; --jn) {
+                                        const int c = q[jn];
+                                        if (c > 3) {
+                                            ok = false;
+                                            break;
+                                        }
+                                        sbytes += lf_pair(a.bwt, M, k - 1, l, ca, cb);
+                                        k = M.L2[c] + ca[c] + 1;
+                                        l = M.L2[c] + cb[c];
+                                        ok = k <= l;
+                                    }
+                                    if (ok && (int)(uint32_t)(l - k + 1) != 0) {
+                                        kind = K_HIT;
+                                        hk = k;
+                                        hl = l;
+                                    }
+                                }
+                            } else {
+                                // expansion: src/bwtgap_hd.h:412-470
+                                sbytes += lfb;
+                                const int i = i0 - 1;
+                                const uint64_t k = e_k(e), l = e_l(e), occ = l - k + 1;
+                                bool allow_diff = true, allow_M = true;
+                                if (i > 0) {
+                                    if (wbid(wA) > m - 1)
+                                        allow_diff = false;
+                                    else if (wbid(wA) == m - 1 && wbid(wB) == m - 1 && wval(wA) == wval(wB))
+                                        allow_M = false;
+                                    if (seeding) {
+                                        const int ii = i - (len - seed_len);
+                                        if (ii > 0) {
+                                            const uint64_t sA = SW[ii - 1], sB = SW[ii];
+                                            if (wbid(sA) > m_seed - 1)
+                                                allow_diff = false;
+                                            else if (wbid(sA) == m_seed - 1 && wbid(sB) == m_seed - 1 && wval(sA) == wval(sB))
+                                                allow_M = false;
+                                        }
+                                    }
+                                }
+                                const int tmp = (mode & MODE_LOGGAP) ? int_log2((uint32_t)(ge + go)) / 2 + 1 : go + ge;
+                                auto logp = [&](int s, const Ent &en) __attribute__((always_inline)) {
+                                    if (kind != K_END) return;
+                                    if (s < 0 || s >= n_stacks) {
+                                        kind = K_ESCORE;
+                                        return;
+                                    }
+                                    PLOG[np] = en;
+                                    PSC[np] = (uint8_t)s;
+                                    ++np;
+                                    ++sp;
+                                };
+                                if (allow_diff && i >= ies + tmp && len - i >= ies + tmp) {
+                                    if (st == ST_M) {
+                                        if (go < max_gapo) {
+                                            const int sc = mm * s_mm + (go + 1) * s_gapo + ge * s_gape;
+                                            logp(sc, mk_ent(k, l, i, i, mm, go + 1, ge, ins + 1, ST_I));
+                                            for (int j = 0; j < 4; ++j) {
+                                                const uint64_t kk = M.L2[j] + ca[j] + 1, ll = M.L2[j] + cb[j];
+                                                if (kk <= ll) logp(sc, mk_ent(kk, ll, i + 1, i + 1, mm, go + 1, ge, ins, ST_D));
+                                            }
+                                        }
+                                    } else if (st == ST_I) {
+                                        if (ge < max_gape)
+                                            logp(mm * s_mm + go * s_gapo + (ge + 1) * s_gape,
+                                                 mk_ent(k, l, i, i, mm, go, ge + 1, ins + 1, ST_I));
+                                    } else {
+                                        if (ge < max_gape && (ge + go < max_diff || occ < (uint64_t)(int64_t)mdo)) {
+                                            const int sc = mm * s_mm + go * s_gapo + (ge + 1) * s_gape;
+                                            for (int j = 0; j < 4; ++j) {
+                                                const uint64_t kk = M.L2[j] + ca[j] + 1, ll = M.L2[j] + cb[j];
+                                                if (kk <= ll) logp(sc, mk_ent(kk, ll, i + 1, i + 1, mm, go, ge + 1, ins, ST_D));
+                                            }
+                                        }
+                                    }
+                                }
+                                if (allow_diff && allow_M) {
+                                    for (int j = 1; j <= 4; ++j) {
+                                        const int c = (qbase + j) & 3;
+                                        const int is_mm = (j != 4 || qbase > 3);
+                                        const uint64_t kk = M.L2[c] + ca[c] + 1, ll = M.L2[c] + cb[c];
+                                        if (kk > ll) continue;
+                                        if (is_mm) {
+                                            logp((mm + 1) * s_mm + go * s_gapo + ge * s_gape,
+                                                 mk_ent(kk, ll, i, i, mm + 1, go, ge, ins, ST_M));
+                                        } else if (kind == K_END) {
+                                            child = mk_ent(kk, ll, i, 0, mm, go, ge, ins, ST_M);
+                                            cont = true;
+                                        }
+                                    }
+                                } else if (qbase < 4) {
+                                    const uint64_t kk = M.L2[qbase] + ca[qbase] + 1, ll = M.L2[qbase] + cb[qbase];
+                                    if (kk <= ll && kind == K_END) {
+                                        child = mk_ent(kk, ll, i, 0, mm, go, ge, ins, ST_M);
+                                        cont = true;
+                                    }
+                                }
+                            }
+                        }
+                    }
+                    SINFO[steps] = sp | sbytes << 8;
+                    ++steps;
+                    if (kind != K_END || !cont) break;
+                    e = child;
+                }
+            }
+            lsteps[lane] = steps;
+            lnp[lane] = np;
+            lkind[lane] = lane < K ? kind : K_END;
+            __syncthreads();
+            const long long t_commit = wall_clock64();
+            // ---- commit plan (lane 0): first stopping chain, push cap, entry counts ----
+            if (lane == 0) {
+                int J = K - 1;
+                for (int j = 0; j < K; ++j)
+                    if (lkind[j] != K_END) {
+                        J = j;
+                        break;
+                    }
+                int pex = 0, jcap = J + 1;
+                long nst = F.n_entries;
+                for (int j = 0; j <= J; ++j) {
+                    lpexcl[j] = pex;
+                    lnstart[j] = (int)nst;
+                    if (pex + lnp[j] > H_CAP && jcap > J) jcap = j;
+                    pex += lnp[j];
+                    nst += lnp[j] - 1 + (lkind[j] == K_TRUNC ? 1 : 0);
+                }
+                F.J = J;
+                F.jcap = jcap;
+            }
+            __syncthreads();
+            // ---- each committed lane: steps it commits, and the max_entries check before each pop ----
+            {
+                const int J = F.J, jcap = F.jcap;
+                int c = 0, cut = -1;
+                if (lane <= J && lane <= jcap) {
+                    c = steps;
+                    if (lane == jcap) {
+                        // the push cap falls inside this chain: commit whole steps up to it
+                        int acc = lpexcl[lane];
+                        c = 0;
+                        while (c < steps && acc + (int)(SINFO[c] & 0xff) <= H_CAP) acc += SINFO[c++] & 0xff;
+                    }
+                    long nb = lnstart[lane];
+                    for (int t = 0; t < c; ++t) {
+                        if (nb > F.max_entries) {
+                            cut = t;
+                            break;
+                        }
+                        nb += SINFO[t] & 0xff;
+                    }
+                }
+                lcommit[lane] = c;
+                lcut[lane] = cut;
+                if (lane <= J && lane <= jcap && c > 0 && (kind == K_HIT || kind == K_ESCORE) && c == steps) {
+                    F.hk = hk;
+                    F.hl = hl;
+                    F.he = SLOG[steps - 1];
+                }
+                if (lane <= J && lane <= jcap && c > 0 && c == steps && kind == K_TRUNC) F.pend = nxt;
+            }
+            __syncthreads();
+            if (lane == 0) {
+                // the last committed chain, and where the search stops inside the round, if it does
+                int last = -1, jcut = -1;
+                for (int j = 0; j <= F.J && j <= F.jcap; ++j) {
+                    if (lcut[j] >= 0) {
+                        jcut = j;
+                        break;
+                    }
+                    if (lcommit[j] > 0) last = j;
+                }
+                F.jcut = jcut;
+                F.tcut = jcut >= 0 ? lcut[jcut] : 0;
+                // pops, trace and algorithmic bytes of the committed steps, in order
+                const int jend = jcut >= 0 ? jcut : last;
+                long nb = F.n_entries;
+                uint64_t tr = F.trace;
+                for (int j = 0; j <= jend; ++j) {
+                    const int cj = (j == jcut) ? F.tcut : lcommit[j];
+                    nb = lnstart[j];
+                    for (int t = 0; t < cj; ++t) {
+                        const uint32_t info = SINFO_ALL[j * H_STEPS + t];
+                        F.obytes += info >> 8;
+                        F.pops += 1;
+                        nb -= 1;
+                        if (TRACE) {
+                            const Ent e = SLOG_ALL[j * H_STEPS + t];
+                            const int i0 = e_i(e), mm = e_mm(e), go = e_go(e), ge = e_ge(e), st = e_st(e), ins = e_ins(e);
+                            tr = tr_mix(tr, (uint64_t)((uint32_t)b << 21 | (uint32_t)i0));
+                            tr = tr_mix(tr, e_k(e));
+                            tr = tr_mix(tr, e_l(e));
+                            tr = tr_mix(tr, (uint64_t)((uint32_t)mm | (uint32_t)go << 8 | (uint32_t)ge << 16 | (uint32_t)st << 24));
+                            tr = tr_mix(tr, (uint64_t)ins | (uint64_t)(go + ge - ins) << 16);
+                            tr = tr_mix(tr, (uint64_t)(uint32_t)e_ldp(e));
+                            tr = tr_mix(tr, (uint64_t)(uint32_t)nb);
+                        }
+                        nb += (info & 0xff) + 1;  // its pushes, and its exact child (popped next, if any)
+                    }
+                }
+                F.trace = tr;
+                if (jcut >= 0) {
+                    F.status = 1;
+                    F.result = F.n_aln;
+                } else {
+                    F.J = last;  // from here on: the committed chains are 0..last
+                    int np_tot = 0;
+                    for (int j = 0; j <= last; ++j) np_tot += lcommit[j] == lsteps[j] ? lnp[j] : 0;
+                    (void)np_tot;
+                }
+            }
+            __syncthreads();
+            if (F.status) break;
+            // ---- pushes: rank of every committed push within its bucket, then segments ----
+            const int last = F.J;
+            int ncp = 0;  // this lane's committed pushes
+            if (lane <= last) {
+                const int c = lcommit[lane];
+                for (int t = 0; t < c; ++t) ncp += SINFO[t] & 0xff;
+            }
+            for (int s = 0; s < NB; ++s) cnt[s * HW + lane] = 0;
+            for (int u = 0; u < ncp; ++u) cnt[PSC[u] * HW + lane] += 1;
+            __syncthreads();
+            for (int s = lane; s < NB; s += HW) {
+                uint32_t run = 0;
+                for (int j = 0; j < HW; ++j) {
+                    const uint32_t v = cnt[s * HW + j];
+                    cnt[s * HW + j] = (uint16_t)run;
+                    run += v;
+                }
+                Tb[s] = run;
+            }
+            __syncthreads();
+            if (lane == 0) {
+                int np_ = 0;
+                bool oom = false;
+                for (int s = 0; s < NB && !oom; ++s) {
+                    uint32_t rem = Tb[s];
+                    if (!rem) continue;
+                    const bool bit = (F.bm[s >> 6] >> (s & 63)) & 1ull;
+                    uint32_t prev_base = bit ? seg_base[s] : NIL, prev_size = bit ? seg_cnt[s] : 0u, r0 = 0;
+                    first_piece[s] = np_;
+                    while (rem > 0) {
+                        if (F.aleft < 2) {
+                            if (F.n_own >= a.own_cap) {
+                                oom = true;
+                                break;
+                            }
+                            const uint32_t g = gran_pop(a.ghead, a.gnext, a.gbump, a.n_gran);
+                            if (g == NIL) {
+                                oom = true;
+                                break;
+                            }
+                            OWN[F.n_own++] = g;
+                            F.abase = g * GENT;
+                            F.aleft = (int)GENT;
+                        }
+                        const uint32_t plen = min(rem, (uint32_t)F.aleft - 1);
+                        Ent hdr;
+                        hdr.w0 = (uint64_t)prev_base | (uint64_t)prev_size << 32;
+                        hdr.w1 = 0;
+                        POOL[F.abase] = hdr;
+                        pieces[np_++] = make_uint4((uint32_t)s, r0, plen, F.abase + 1);
+                        prev_base = F.abase + 1;
+                        prev_size = plen;
+                        F.abase += plen + 1;
+                        F.aleft -= (int)plen + 1;
+                        rem -= plen;
+                        r0 += plen;
+                    }
+                    seg_base[s] = prev_base;
+                    seg_cnt[s] = prev_size;
+                    F.bm[s >> 6] |= 1ull << (s & 63);
+                }
+                if (oom) {
+                    F.status = 1;
+                    F.result = ERR_STACK;
+                }
+                F.n_piece = np_;
+            }
+            __syncthreads();
+            if (F.status) break;
+            for (int u = 0; u < ncp; ++u) {
+                const int s = PSC[u];
+                const uint32_t r = cnt[s * HW + lane];
+                cnt[s * HW + lane] = (uint16_t)(r + 1);
+                int p = first_piece[s];
+                while (r >= pieces[p].y + pieces[p].z) ++p;
+                POOL[pieces[p].w + (r - pieces[p].y)] = PLOG[u];
+            }
+            __syncthreads();
+            // ---- bucket b: the committed start entries leave its top; the state moves on ----
+            if (lane == 0) {
+                int taken = 0;
+                for (int j = 0; j <= last; ++j) taken += lcommit[j] > 0;
+                const int from_seg = taken - (F.has_pend && taken > 0 ? 1 : 0);
+                if (taken > 0) F.has_pend = 0;
+                if (from_seg > 0) {
+                    seg_cnt[b] -= (uint32_t)from_seg;
+                    if (seg_cnt[b] == 0) {
+                        const Ent hdr = POOL[seg_base[b] - 1];
+                        const uint32_t pb = (uint32_t)hdr.w0, ps = (uint32_t)(hdr.w0 >> 32);
+                        if (pb == NIL) {
+                            F.bm[b >> 6] &= ~(1ull << (b & 63));
+                        } else {
+                            seg_base[b] = pb;
+                            seg_cnt[b] = ps;
+                        }
+                    }
+                }
+                // entries: every committed pop -1, every committed push +1, a pending child +1
+                long ne = F.n_entries;
+                for (int j = 0; j <= last; ++j) {
+                    const int c = lcommit[j];
+                    int pj = 0;
+                    for (int t = 0; t < c; ++t) pj += SINFO_ALL[j * H_STEPS + t] & 0xff;
+                    ne += pj - 1;
+                }
+                const int lc = last >= 0 ? lcommit[last] : 0;
+                const bool partial = last >= 0 && lc < lsteps[last];
+                const bool trunc = last >= 0 && lc == lsteps[last] && lkind[last] == K_TRUNC;
+                if (partial) F.pend = SLOG_ALL[last * H_STEPS + lc];
+                if (partial || trunc) {
+                    F.has_pend = 1;
+                    ne += 1;
+                }
+                F.n_entries = (int)ne;
+                if (a.stats) {
+                    int cs = 0, mx = 0;
+                    for (int j = 0; j < HW; ++j) mx = max(mx, lsteps[j]);
+                    for (int j = 0; j <= last; ++j) cs += lcommit[j];
+                    atomicAdd(a.stats + 0, 1ull);
+                    atomicAdd(a.stats + 1, (unsigned long long)F.K);
+                    atomicAdd(a.stats + 2, (unsigned long long)(last + 1));
+                    atomicAdd(a.stats + 3, (unsigned long long)cs);
+                    atomicAdd(a.stats + 4, (unsigned long long)mx);
+                    atomicAdd(a.stats + 5, (unsigned long long)(t_commit - t_spec));
+                    atomicAdd(a.stats + 6, (unsigned long long)(wall_clock64() - t_commit));
+                }
+                // a hit or a score error ends the last committed chain
+                const bool full = last >= 0 && lc == lsteps[last];
+                if (full && lkind[last] == K_ESCORE) {
+                    F.status = 1;
+                    F.result = ERR_SCORE;
+                } else if (full && lkind[last] == K_HIT) {
+                    const Ent e = F.he;
+                    const uint64_t k = F.hk, l = F.hl;
+                    const int mm = e_mm(e), go = e_go(e), ge = e_ge(e);
+                    const int score = b;
+                    bool do_add = true, stop = false;
+                    if (F.n_aln == 0) {
+                        F.best_score = score;
+                        F.best_diff = mm + go + ((F.mode & MODE_GAPE) ? ge : 0);
+                        if (!(F.mode & MODE_NONSTOP))
+                            F.max_diff = (F.best_diff + 1 > F.md_read) ? F.md_read : F.best_diff + 1;
+                    }
+                    if (score == F.best_score) {
+                        F.best_cnt = (int)(uint32_t)((uint64_t)(int64_t)F.best_cnt + (l - k + 1));
+                    } else if (F.best_cnt > F.max_top2) {
+                        stop = true;
+                    }
+                    if (!stop) {
+                        if (go) {
+                            for (int j = 0; j < F.n_aln; ++j)
+                                if (REC[j].k == k && REC[j].l == l) {
+                                    do_add = false;
+                                    break;
+                                }
+                        }
+                        if (do_add) {
+                            const int x = (int)(uint32_t)(l - k + 1);
+                            const uint64_t ux = (uint64_t)(int64_t)x;
+                            int jj = 0;
+                            for (int i = 0; i < e_ldp(e); ++i) {
+                                uint64_t w = wval(W[i]);
+                                int bid = wbid(W[i]);
+                                if (w > ux) w -= ux;
+                                else if (w == ux) {
+                                    bid = 1;
+                                    w = seq_len - (uint64_t)(++jj);
+                                } else continue;
+                                W[i] = wpack(w, bid);
+                            }
+                            if (F.n_aln == a.rec_cap) {
+                                F.status = 1;
+                                F.result = ERR_ALN;
+                            } else {
+                                const int ins = e_ins(e), del = go + ge - ins;
+                                galn_aln_rec_t r;
+                                r.packed = ((uint64_t)(mm & 0xff)) | ((uint64_t)(go & 0xff) << 8) |
+                                           ((uint64_t)(ge & 0xff) << 16) | ((uint64_t)(score & 0xfffff) << 24) |
+                                           ((uint64_t)(ins & 0x3ff) << 44) | ((uint64_t)(del & 0x3ff) << 54);
+                                r.k = k;
+                                r.l = l;
+                                REC[F.n_aln] = r;
+                                F.n_aln += 1;
+                            }
+                        }
+                    } else {
+                        F.status = 1;
+                        F.result = F.n_aln;
+                    }
+                }
+            }
+            __syncthreads();
+        }
+        // ---- the read's result ----
+        if (lane == 0) {
+            int n = F.result;
+            const int rd = F.rid;
+            if (n > 0) {
+                const unsigned long long b0 = atomicAdd(a.out_top, (unsigned long long)n);
+                if (b0 + (unsigned long long)n > a.out_cap) {
+                    n = ERR_OUT;
+                } else {
+                    for (int j = 0; j < n; ++j) a.out_rec[b0 + j] = REC[j];
+                    a.out_off[rd] = b0;
+                }
+            }
+            if (n == ERR_STACK || n == ERR_ALN || n == ERR_OUT) {
+                const int f = atomicAdd(a.fail_n, 1);
+                a.fail_list[f] = rd;
+                atomicOr(a.fail_kind + (n == ERR_OUT ? 0 : n == ERR_ALN ? 1 : 2), 1);
+            } else {
+                atomicAdd(a.algo_bytes, F.obytes);
+            }
+            a.out_n[rd] = n;
+            a.out_pops[rd] = F.pops;
+            if (TRACE) a.out_trace[rd] = F.trace;
+            if (a.out_bytes) a.out_bytes[rd] = F.obytes;
+            if (F.n_own > 0) gran_push(a.ghead, a.gnext, OWN, 0, F.n_own);
+            F.n_own = 0;
+        }
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------------------
 // host side
 
 thread_local char g_err[512];
@@ -813,9 +1651,8 @@ int fail(int code, const char *fmt, ...)
     } while (0)
 
 struct Tier {
-    int n_chunks;
-    int rec_cap;
-    bool wide;  // u32 bucket tops
+    int rec_cap;   // record slots per lane
+    int keep;      // granules a lane keeps between reads
 };
 
 }  // namespace
@@ -827,7 +1664,11 @@ struct galn_hip_index {
     DMeta meta{};
     galn_bwt_meta_t hmeta{};
     int n_cu = 0;
-    size_t lds_per_block_max = 65536;
+    // the stack pool (grown on demand, shared by the searches on this index)
+    Ent *d_pool_chk = nullptr;
+    uint32_t *d_pool_link = nullptr, *d_gnext = nullptr, *d_gbump = nullptr;
+    unsigned long long *d_ghead = nullptr;
+    uint32_t n_gran = 0;
 };
 
 struct galn_hip_readset {
@@ -837,6 +1678,7 @@ struct galn_hip_readset {
     int seed_len = 0;
     int max_len = 0;
     int max_stacks = 0;
+    int max_entries = 0;
     uint64_t qbytes = 0;
     // device inputs
     uint8_t *d_query = nullptr;
@@ -846,22 +1688,36 @@ struct galn_hip_readset {
     int32_t *d_sorted = nullptr;   // read ids, longest first
     // device outputs
     int32_t *d_out_n = nullptr;
-    uint64_t *d_out_off = nullptr, *d_pops = nullptr, *d_trace = nullptr;
+    uint64_t *d_out_off = nullptr, *d_pops = nullptr, *d_trace = nullptr, *d_bytes = nullptr;
     galn_aln_rec_t *d_rec = nullptr;
     uint64_t rec_cap = 0;
     unsigned long long *d_top = nullptr;
     int32_t *d_fail[2] = {nullptr, nullptr};
-    int32_t *d_ctr = nullptr;      // [0] fail count A, [1] fail count B, [2] queue head
-    // scratch arena (grown on demand)
+    int32_t *d_ctr = nullptr;      // [0],[1] failure counts, [2] queue head, [3] heavy count, [4..6] failure kinds
+    int32_t *d_heavy = nullptr;    // reads handed to the wave-per-read engine
+    unsigned long long *d_stats = nullptr;  // wave-per-read engine counters of the last search
+    uint8_t *d_hscratch = nullptr; // per-wave regions of the wave-per-read engine
+    size_t hscratch_bytes = 0;
+    // per-lane regions (grown on demand)
     uint8_t *d_scratch = nullptr;
     size_t scratch_bytes = 0;
     // host copies
     std::vector<int32_t> h_qlen, h_nstacks, h_batch;
     std::vector<galn_search_opt_t> h_bopts;
-    // last search
+    // last search: one record per kernel launch, timed with events on the search stream
     double first_ms = 0, total_ms = 0;
     uint32_t last_flags = 0;
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+    struct Launch {
+        int tier;
+        int slot;
+        double ms;
+        uint64_t bytes;
+    };
+    std::vector<Launch> launches;
+    std::vector<hipEvent_t> lev;           // 2 per launch, reused across searches
+    unsigned long long *d_algo = nullptr;  // per-launch algorithmic byte counters
+    static constexpr int MAX_LAUNCH = 256;
 };
 
 namespace {
@@ -870,10 +1726,11 @@ size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
 
 struct Layout {
     uint64_t slot_bytes;
-    uint32_t sw_off, rec_off, link_off, chunk_off;
+    uint32_t sw_off, rec_off, own_off;
+    int own_cap;
 };
 
-Layout make_layout(int max_len, int seed_len, int n_chunks, int rec_cap)
+Layout make_layout(int max_len, int seed_len, int rec_cap, int own_cap)
 {
     Layout L{};
     size_t off = align_up((size_t)(max_len + 1) * 8, 256);
@@ -882,14 +1739,12 @@ Layout make_layout(int max_len, int seed_len, int n_chunks, int rec_cap)
     off += align_up((size_t)(sl + 1) * 8, 256);
     L.rec_off = (uint32_t)off;
     off += align_up((size_t)rec_cap * sizeof(galn_aln_rec_t), 256);
-    L.link_off = (uint32_t)off;
-    off += align_up((size_t)n_chunks * 4, 256);
-    L.chunk_off = (uint32_t)off;
-    off += (size_t)n_chunks * CH * sizeof(Ent);
-    off = align_up(off, 256);
-    // an odd multiple of 256 bytes keeps lanes' arenas off the same cache sets
+    L.own_off = (uint32_t)off;
+    off += align_up((size_t)own_cap * 4, 256);
+    // an odd multiple of 256 bytes keeps lanes' regions off the same cache sets
     if ((off / 256) % 2 == 0) off += 256;
     L.slot_bytes = off;
+    L.own_cap = own_cap;
     return L;
 }
 
@@ -904,30 +1759,112 @@ int ensure_scratch(galn_hip_readset *rs, size_t bytes)
     return 0;
 }
 
-template <bool TRACE, bool PERSIST, typename TopT>
+// The stack pool: `want` granules of 64 chunks (16 KB each), within half the free memory.
+int ensure_pool(galn_hip_index *idx, uint64_t want)
+{
+    const uint64_t gbytes = (uint64_t)GRAN * CH * sizeof(Ent);
+    size_t freeb = 0, totalb = 0;
+    HIP_OK(hipMemGetInfo(&freeb, &totalb));
+    const uint64_t have = idx->n_gran;
+    uint64_t cap = (uint64_t)((freeb + have * (gbytes + GRAN * 4 + 4)) / 2) / (gbytes + GRAN * 4 + 4);
+    cap = std::min<uint64_t>(cap, ((uint64_t)48 << 30) / gbytes);
+    cap = std::min<uint64_t>(cap, MAX_POOL_CHUNKS / GRAN);
+    want = std::min(std::max<uint64_t>(want, 1024), cap);
+    if (have >= want) return 0;
+    if (idx->d_pool_chk) {
+        HIP_OK(hipFree(idx->d_pool_chk));
+        HIP_OK(hipFree(idx->d_pool_link));
+        HIP_OK(hipFree(idx->d_gnext));
+        idx->d_pool_chk = nullptr;
+        idx->n_gran = 0;
+    }
+    if (!idx->d_ghead) {
+        HIP_OK(hipMalloc(&idx->d_ghead, 8));
+        HIP_OK(hipMalloc(&idx->d_gbump, 4));
+    }
+    HIP_OK(hipMalloc(&idx->d_pool_chk, want * gbytes));
+    HIP_OK(hipMalloc(&idx->d_pool_link, want * GRAN * 4));
+    HIP_OK(hipMalloc(&idx->d_gnext, want * 4));
+    idx->n_gran = (uint32_t)want;
+    return 0;
+}
+
+template <bool TRACE, bool PERSIST>
 int launch_one(galn_hip_index *idx, const KArgs &a, int block, int grid, int buckets)
 {
-    const size_t lds = (size_t)block * buckets * sizeof(TopT);
-    hipLaunchKernelGGL((k_search<TRACE, PERSIST, TopT>), dim3(grid), dim3(block), lds, idx->stream, a);
+    const size_t lds = (size_t)block * buckets * sizeof(uint32_t);
+    hipLaunchKernelGGL((k_search<TRACE, PERSIST>), dim3(grid), dim3(block), lds, idx->stream, a);
     HIP_OK(hipGetLastError());
     return 0;
 }
 
-template <bool TRACE>
-int launch_t(galn_hip_index *idx, const KArgs &a, int block, int grid, bool persist, bool wide, int buckets)
+int launch(galn_hip_index *idx, const KArgs &a, int block, int grid, bool persist, bool trace, int buckets)
 {
-    if (persist) {
-        return wide ? launch_one<TRACE, true, uint32_t>(idx, a, block, grid, buckets)
-                    : launch_one<TRACE, true, uint16_t>(idx, a, block, grid, buckets);
-    }
-    return wide ? launch_one<TRACE, false, uint32_t>(idx, a, block, grid, buckets)
-                : launch_one<TRACE, false, uint16_t>(idx, a, block, grid, buckets);
+    if (trace) return persist ? launch_one<true, true>(idx, a, block, grid, buckets)
+                              : launch_one<true, false>(idx, a, block, grid, buckets);
+    return persist ? launch_one<false, true>(idx, a, block, grid, buckets)
+                   : launch_one<false, false>(idx, a, block, grid, buckets);
 }
 
-int launch(galn_hip_index *idx, const KArgs &a, int block, int grid, bool persist, bool wide, bool trace, int buckets)
+int buckets_of(const galn_hip_readset *rs) { return std::max(1, std::min(rs->max_stacks, MAXB)); }
+
+// threads per workgroup: the bucket tops of a workgroup must fit 64 KB of LDS
+int pick_block(const galn_hip_readset *rs) { return buckets_of(rs) * 128 * 4 <= 65536 ? 128 : 64; }
+
+// persistent grid: the workgroups that are resident at once
+int resident_slots(galn_hip_readset *rs, int block, bool trace, int waves_per_cu)
 {
-    return trace ? launch_t<true>(idx, a, block, grid, persist, wide, buckets)
-                 : launch_t<false>(idx, a, block, grid, persist, wide, buckets);
+    galn_hip_index *idx = rs->idx;
+    if (waves_per_cu > 0) return std::max(block, idx->n_cu * waves_per_cu * 64 / block * block);
+    int nb = 0;
+    const size_t lds = (size_t)block * buckets_of(rs) * 4;
+    hipError_t e = trace ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_search<true, true>, block, lds)
+                         : hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_search<false, true>, block, lds);
+    if (e != hipSuccess || nb < 1) nb = 1;
+    return nb * idx->n_cu * block;
+}
+
+struct HLayout {
+    uint64_t slot_bytes;
+    uint32_t rec_off, own_off, plog_off, pscore_off, slog_off, sinfo_off;
+};
+
+HLayout make_hlayout(int rec_cap, int own_cap)
+{
+    HLayout L{};
+    size_t off = 0;
+    L.rec_off = 0;
+    off += align_up((size_t)rec_cap * sizeof(galn_aln_rec_t), 256);
+    L.own_off = (uint32_t)off;
+    off += align_up((size_t)own_cap * 4, 256);
+    L.plog_off = (uint32_t)off;
+    off += (size_t)HW * H_PUSH * sizeof(Ent);
+    L.pscore_off = (uint32_t)off;
+    off += align_up((size_t)HW * H_PUSH, 256);
+    L.slog_off = (uint32_t)off;
+    off += (size_t)HW * H_STEPS * sizeof(Ent);
+    L.sinfo_off = (uint32_t)off;
+    off += align_up((size_t)HW * H_STEPS * 4, 256);
+    if ((off / 256) % 2 == 0) off += 256;
+    L.slot_bytes = off;
+    return L;
+}
+
+size_t heavy_lds_bytes(int buckets, int wlen, int swlen)
+{
+    size_t b = (sizeof(HShared) + 15) & ~(size_t)15;
+    b += (size_t)buckets * 16 + (size_t)buckets * HW * 2;
+    b = (b + 15) & ~(size_t)15;
+    b += (size_t)H_PIECES * 16 + (size_t)(wlen + swlen) * 8 + 7 * HW * 4;
+    return b;
+}
+
+// the wave-per-read engine keeps each chain's pops in order only when every child scores above its parent
+bool heavy_ok(const galn_hip_readset *rs)
+{
+    for (const auto &o : rs->h_bopts)
+        if (o.s_mm <= 0 || o.s_gapo <= 0 || o.s_gape <= 0) return false;
+    return true;
 }
 
 KArgs base_args(galn_hip_readset *rs, uint32_t flags)
@@ -948,9 +1885,16 @@ KArgs base_args(galn_hip_readset *rs, uint32_t flags)
     a.out_off = rs->d_out_off;
     a.out_pops = rs->d_pops;
     a.out_trace = rs->d_trace;
+    a.out_bytes = rs->d_bytes;
     a.out_rec = rs->d_rec;
     a.out_cap = rs->rec_cap;
     a.out_top = rs->d_top;
+    a.pool_chk = idx->d_pool_chk;
+    a.pool_link = idx->d_pool_link;
+    a.gnext = idx->d_gnext;
+    a.ghead = idx->d_ghead;
+    a.gbump = idx->d_gbump;
+    a.n_gran = idx->n_gran;
     // keeping the exact child in registers relies on children never scoring below their parent
     bool nonneg = true;
     for (const auto &o : rs->h_bopts) nonneg = nonneg && o.s_mm >= 0 && o.s_gapo >= 0 && o.s_gape >= 0;
@@ -958,31 +1902,21 @@ KArgs base_args(galn_hip_readset *rs, uint32_t flags)
     return a;
 }
 
-// Runs reads list[0..n) (or identity from list_base) in one tier; failures land in fail buffer `fo`.
+// One launch over list positions [0, n_list) (read id = list[pos], or list_base + pos);
+// failures land in fail buffer `fo`.
 int run_tier(galn_hip_readset *rs, uint32_t flags, const Tier &t, const int32_t *list, int list_base, int n_list,
-             bool persist, int max_slots, int fo, int block)
+             bool persist, int slots, int fo, int block, int tier_id, uint64_t pop_budget = 0)
 {
     galn_hip_index *idx = rs->idx;
     if (n_list <= 0) return 0;
-    Layout L = make_layout(rs->max_len, rs->seed_len, t.n_chunks, t.rec_cap);
-    int slots = persist ? std::min(max_slots, n_list) : n_list;
-    // keep the arena within a budget: at most 24 GB of lane arenas per launch
-    const size_t budget = (size_t)24 << 30;
-    if ((size_t)slots * L.slot_bytes > budget) {
-        const int cap = (int)std::max<size_t>(block, budget / L.slot_bytes / block * block);
-        if (!persist) {
-            // split a plain launch into pieces that fit
-            for (int s = 0; s < n_list; s += cap) {
-                const int r = run_tier(rs, flags, t, list ? list + s : nullptr, list_base + (list ? 0 : s),
-                                       std::min(cap, n_list - s), false, max_slots, fo, block);
-                if (r) return r;
-            }
-            return 0;
-        }
-        slots = std::min(slots, cap);
-    }
+    if (!persist) slots = n_list;
+    slots = std::max(1, std::min(slots, n_list));
+    const int own_cap = (int)(((int64_t)(rs->max_entries + 16) / CH + MAXB + 2) / GRAN + 2);
+    Layout L = make_layout(rs->max_len, rs->seed_len, t.rec_cap, own_cap);
     int r = ensure_scratch(rs, (size_t)slots * L.slot_bytes);
     if (r) return r;
+    // pool: what the lanes keep plus room for the heavy reads in flight
+    if ((r = ensure_pool(idx, (uint64_t)slots * (uint64_t)(t.keep + 12)))) return r;
     KArgs a = base_args(rs, flags);
     a.list = list;
     a.list_base = list_base;
@@ -993,17 +1927,137 @@ int run_tier(galn_hip_readset *rs, uint32_t flags, const Tier &t, const int32_t 
     a.slot_bytes = L.slot_bytes;
     a.sw_off = L.sw_off;
     a.rec_off = L.rec_off;
-    a.link_off = L.link_off;
-    a.chunk_off = L.chunk_off;
-    a.n_chunks = t.n_chunks;
+    a.own_off = L.own_off;
+    a.own_cap = L.own_cap;
+    a.keep = t.keep;
     a.rec_cap = t.rec_cap;
     a.fail_list = rs->d_fail[fo];
     a.fail_n = rs->d_ctr + fo;
     a.fail_kind = rs->d_ctr + 4;
+    a.pop_budget = pop_budget;
+    a.heavy_list = rs->d_heavy;
+    a.heavy_n = rs->d_ctr + 3;
     if (persist) HIP_OK(hipMemsetAsync(rs->d_ctr + 2, 0, sizeof(int32_t), idx->stream));
+    // an empty free stack (tag and granule all ones) and no granule handed out yet
+    HIP_OK(hipMemsetAsync(idx->d_ghead, 0xff, 8, idx->stream));
+    HIP_OK(hipMemsetAsync(idx->d_gbump, 0, 4, idx->stream));
+    const int li = (int)rs->launches.size();
+    if (li >= galn_hip_readset::MAX_LAUNCH) return fail(GALN_HIP_E_DEVICE, "too many launches in one search");
+    while ((int)rs->lev.size() < 2 * (li + 1)) {
+        hipEvent_t e;
+        HIP_OK(hipEventCreate(&e));
+        rs->lev.push_back(e);
+    }
+    a.algo_bytes = rs->d_algo + li;
+    rs->launches.push_back({tier_id, li, 0.0, 0});
     const int grid = (slots + block - 1) / block;
-    return launch(idx, a, block, grid, persist, t.wide, (flags & GALN_HIP_TRACE) != 0,
-                  std::max(1, std::min(rs->max_stacks, MAXB)));
+    HIP_OK(hipEventRecord(rs->lev[2 * li], idx->stream));
+    r = launch(idx, a, block, grid, persist, (flags & GALN_HIP_TRACE) != 0, buckets_of(rs));
+    if (r) return r;
+    HIP_OK(hipEventRecord(rs->lev[2 * li + 1], idx->stream));
+    return 0;
+}
+
+// The wave-per-read engine over list[0..n): one wave per read, `waves` resident waves
+// claiming reads from a device counter; failures land in fail buffer `fo`.
+int run_heavy(galn_hip_readset *rs, uint32_t flags, const int32_t *list, int n, int rec_cap, int fo, int tier_id,
+              int waves)
+{
+    galn_hip_index *idx = rs->idx;
+    if (n <= 0) return 0;
+    waves = std::max(1, std::min(waves, n));
+    int r = ensure_pool(idx, (uint64_t)waves * 256);
+    if (r) return r;
+    const int own_cap = (int)std::min<uint64_t>(idx->n_gran, 32768);
+    const HLayout L = make_hlayout(rec_cap, own_cap);
+    if (rs->hscratch_bytes < (size_t)waves * L.slot_bytes) {
+        if (rs->d_hscratch) HIP_OK(hipFree(rs->d_hscratch));
+        rs->d_hscratch = nullptr;
+        rs->hscratch_bytes = 0;
+        HIP_OK(hipMalloc(&rs->d_hscratch, (size_t)waves * L.slot_bytes));
+        rs->hscratch_bytes = (size_t)waves * L.slot_bytes;
+    }
+    const int wlen = rs->max_len + 1;
+    const int swlen = (rs->seed_len > 0 && rs->seed_len < rs->max_len) ? rs->seed_len + 1 : 1;
+    const size_t lds = heavy_lds_bytes(buckets_of(rs), wlen, swlen);
+    if (lds > 65536) return fail(GALN_HIP_E_UNSUPPORTED, "wave engine: %zu bytes of LDS needed", lds);
+    HArgs a{};
+    a.bwt = idx->d_bwt;
+    a.meta = idx->meta;
+    a.query = rs->d_query;
+    a.qoff = rs->d_qoff;
+    a.qlen = rs->d_qlen;
+    a.maxdiff = rs->d_maxdiff;
+    a.batch = rs->d_batch;
+    a.bopts = rs->d_bopts;
+    a.nstacks = rs->d_nstacks;
+    a.seed_len = rs->seed_len;
+    a.list = list;
+    a.n_list = n;
+    a.qnext = rs->d_ctr + 2;
+    a.n_slots = waves;
+    a.scratch = rs->d_hscratch;
+    a.slot_bytes = L.slot_bytes;
+    a.rec_off = L.rec_off;
+    a.own_off = L.own_off;
+    a.plog_off = L.plog_off;
+    a.pscore_off = L.pscore_off;
+    a.slog_off = L.slog_off;
+    a.sinfo_off = L.sinfo_off;
+    a.own_cap = own_cap;
+    a.rec_cap = rec_cap;
+    a.pool = idx->d_pool_chk;
+    a.gnext = idx->d_gnext;
+    a.ghead = idx->d_ghead;
+    a.gbump = idx->d_gbump;
+    a.n_gran = idx->n_gran;
+    a.out_n = rs->d_out_n;
+    a.out_off = rs->d_out_off;
+    a.out_pops = rs->d_pops;
+    a.out_trace = rs->d_trace;
+    a.out_bytes = rs->d_bytes;
+    a.out_rec = rs->d_rec;
+    a.out_cap = rs->rec_cap;
+    a.out_top = rs->d_top;
+    a.fail_list = rs->d_fail[fo];
+    a.fail_n = rs->d_ctr + fo;
+    a.fail_kind = rs->d_ctr + 4;
+    a.buckets = buckets_of(rs);
+    a.wlen = wlen;
+    a.swlen = swlen;
+    a.stats = rs->d_stats;
+    HIP_OK(hipMemsetAsync(rs->d_ctr + 2, 0, sizeof(int32_t), idx->stream));
+    HIP_OK(hipMemsetAsync(idx->d_ghead, 0xff, 8, idx->stream));
+    HIP_OK(hipMemsetAsync(idx->d_gbump, 0, 4, idx->stream));
+    const int li = (int)rs->launches.size();
+    if (li >= galn_hip_readset::MAX_LAUNCH) return fail(GALN_HIP_E_DEVICE, "too many launches in one search");
+    while ((int)rs->lev.size() < 2 * (li + 1)) {
+        hipEvent_t e;
+        HIP_OK(hipEventCreate(&e));
+        rs->lev.push_back(e);
+    }
+    a.algo_bytes = rs->d_algo + li;
+    rs->launches.push_back({tier_id, li, 0.0, 0});
+    HIP_OK(hipEventRecord(rs->lev[2 * li], idx->stream));
+    if (flags & GALN_HIP_TRACE)
+        hipLaunchKernelGGL(k_heavy<true>, dim3(waves), dim3(HW), lds, idx->stream, a);
+    else
+        hipLaunchKernelGGL(k_heavy<false>, dim3(waves), dim3(HW), lds, idx->stream, a);
+    HIP_OK(hipGetLastError());
+    HIP_OK(hipEventRecord(rs->lev[2 * li + 1], idx->stream));
+    return 0;
+}
+
+int heavy_waves(galn_hip_readset *rs, uint32_t flags)
+{
+    const int wlen = rs->max_len + 1;
+    const int swlen = (rs->seed_len > 0 && rs->seed_len < rs->max_len) ? rs->seed_len + 1 : 1;
+    const size_t lds = heavy_lds_bytes(buckets_of(rs), wlen, swlen);
+    int nb = 0;
+    hipError_t e = (flags & GALN_HIP_TRACE) ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_heavy<true>, HW, lds)
+                                            : hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_heavy<false>, HW, lds);
+    if (e != hipSuccess || nb < 1) nb = 1;
+    return nb * rs->idx->n_cu;
 }
 
 int read_ctr(galn_hip_readset *rs, int which, int32_t *v)
@@ -1034,44 +2088,34 @@ int grow_records(galn_hip_readset *rs)
     return 0;
 }
 
-int pick_block(int max_stacks, bool wide)
-{
-    (void)max_stacks;
-    (void)wide;
-    return 128;
-}
-
-// Overflow tiers: re-run failures (fail buffer `fi`, count n) until none is left.
-int run_overflow(galn_hip_readset *rs, uint32_t flags, int fi, int32_t n)
+// Re-run failed reads (fail buffer `fi`, count n) until none is left: a full output arena
+// grows, full record slots get 16 times more, an exhausted pool gets fewer lanes sharing it.
+int run_retries(galn_hip_readset *rs, uint32_t flags, int fi, int32_t n, Tier t, int slots, uint64_t budget,
+                int hwaves)
 {
     galn_hip_index *idx = rs->idx;
-    int max_entries = 0;
-    for (const auto &o : rs->h_bopts) max_entries = std::max(max_entries, o.max_entries);
-    const int big_chunks = (int)std::min<long>((long)(max_entries + 16) / CH + MAXB + 2, (1l << 27) - 1);
-    Tier tiers[3] = {{1024, 256, false}, {4095, 1024, false}, {big_chunks, 4096, true}};
-    int level = 0;
-    int rec_boost = 1;
-    while (n > 0) {
+    const int block = pick_block(rs);
+    for (int round = 0; n > 0; ++round) {
+        if (round >= 24) return fail(GALN_HIP_E_NOMEM, "reads still failing after %d re-runs", round);
+        int32_t kind[3] = {0, 0, 0};
+        HIP_OK(hipMemcpy(kind, rs->d_ctr + 4, sizeof(kind), hipMemcpyDeviceToHost));
+        int r;
+        if (kind[0] && (r = grow_records(rs))) return r;
+        if (kind[1]) t.rec_cap *= 16;
+        if (kind[2]) {
+            slots = std::max(block, slots / 4);
+            hwaves = std::max(1, hwaves / 4);
+        }
+        slots = std::min(slots, std::max(block, (int)n));
         const int fo = fi ^ 1;
         HIP_OK(hipMemsetAsync(rs->d_ctr + fo, 0, sizeof(int32_t), idx->stream));
-        HIP_OK(hipMemsetAsync(rs->d_ctr + 4, 0, 3 * sizeof(int32_t), idx->stream));
-        Tier t = tiers[std::min(level, 2)];
-        t.rec_cap *= rec_boost;
-        const int slots = level == 0 ? idx->n_cu * 256 : level == 1 ? idx->n_cu * 64 : std::max(64, idx->n_cu * 2);
-        int r = run_tier(rs, flags, t, rs->d_fail[fi], 0, n, true, slots, fo, 64);
-        if (r) return r;
+        HIP_OK(hipMemsetAsync(rs->d_ctr + 3, 0, 4 * sizeof(int32_t), idx->stream));
+        if ((r = run_tier(rs, flags, t, rs->d_fail[fi], 0, n, true, slots, fo, block, 1 + round, budget))) return r;
+        int32_t nh = 0;
+        if ((r = read_ctr(rs, 3, &nh))) return r;
+        if (nh > 0 && (r = run_heavy(rs, flags, rs->d_heavy, nh, 256 * (1 + round), fo, 1 + round, hwaves))) return r;
         int32_t nf = 0;
         if ((r = read_ctr(rs, fo, &nf))) return r;
-        if (nf > 0) {
-            // classify the failures: output arena full -> grow it; records -> more slots
-            int32_t kind[3] = {0, 0, 0};
-            HIP_OK(hipMemcpy(kind, rs->d_ctr + 4, sizeof(kind), hipMemcpyDeviceToHost));
-            const bool any_out = kind[0] != 0, any_aln = kind[1] != 0, any_stack = kind[2] != 0;
-            if (any_out && (r = grow_records(rs))) return r;
-            if (any_stack) ++level;
-            else if (any_aln) rec_boost *= 4;
-            if (level > 2 && any_stack) return fail(GALN_HIP_E_NOMEM, "stack arena still too small at the last tier");
-        }
         fi = fo;
         n = nf;
     }
@@ -1083,43 +2127,55 @@ int readset_search(galn_hip_readset *rs, int form, uint32_t flags, const galn_hi
     galn_hip_index *idx = rs->idx;
     HIP_OK(hipSetDevice(idx->device));
     rs->last_flags = flags;
+    rs->launches.clear();
+    HIP_OK(hipMemsetAsync(rs->d_algo, 0, sizeof(unsigned long long) * galn_hip_readset::MAX_LAUNCH, idx->stream));
+    HIP_OK(hipMemsetAsync(rs->d_stats, 0, 16 * 8, idx->stream));
     HIP_OK(hipMemsetAsync(rs->d_top, 0, sizeof(unsigned long long), idx->stream));
     HIP_OK(hipMemsetAsync(rs->d_ctr, 0, 8 * sizeof(int32_t), idx->stream));
-    const int block = pick_block(rs->max_stacks, false);
-    Tier t0{64, 32, false};
-    if (tune && tune->chunks_per_lane > 0) t0.n_chunks = std::min(tune->chunks_per_lane, 4095);
+    const bool trace = (flags & GALN_HIP_TRACE) != 0;
+    int block = pick_block(rs);
+    Tier t0{64, 4};
     if (tune && tune->rec_per_lane > 0) t0.rec_cap = tune->rec_per_lane;
-    HIP_OK(hipEventRecord(rs->ev[0], idx->stream));
-    int r = 0;
+    if (tune && tune->chunks_per_lane > 0) t0.keep = std::max(1, tune->chunks_per_lane / GRAN);
+    int r = 0, slots = 0, hwaves = 0;
+    uint64_t budget = 0;
     if (form == 0) {
-        // plain: one launch per 0x40000 reads, input order
+        // plain: one launch per 0x40000 reads, one lane per read, input order
+        slots = 0x40000;
         for (int s = 0; s < rs->n && !r; s += 0x40000)
-            r = run_tier(rs, flags, t0, nullptr, s, std::min(0x40000, rs->n - s), false, 0, 0, block);
+            r = run_tier(rs, flags, t0, nullptr, s, std::min(0x40000, rs->n - s), false, 0, 0, block, 0);
     } else {
-        int waves = (tune && tune->waves_per_cu > 0) ? tune->waves_per_cu : 16;
-        const int blk = (tune && tune->block_threads > 0) ? tune->block_threads : block;
-        const int slots = idx->n_cu * waves * 64;
+        // batched: one persistent launch over the reads, longest first; reads that spend their
+        // pop budget go to the wave-per-read engine
+        if (tune && tune->block_threads > 0 && buckets_of(rs) * tune->block_threads * 4 <= 65536)
+            block = tune->block_threads;
+        slots = resident_slots(rs, block, trace, tune ? tune->waves_per_cu : 0);
         const bool sorted = !tune || tune->sort_by_cost != 0;
-        r = run_tier(rs, flags, t0, sorted ? rs->d_sorted : nullptr, 0, rs->n, true, slots, 0, blk);
+        const int hp = tune ? tune->heavy_pops : 0;
+        budget = (hp < 0 || !heavy_ok(rs)) ? 0 : (uint64_t)(hp > 0 ? hp : 4096);
+        hwaves = heavy_waves(rs, flags);
+        r = run_tier(rs, flags, t0, sorted ? rs->d_sorted : nullptr, 0, rs->n, true, slots, 0, block, 0, budget);
+        int32_t nh = 0;
+        if (!r && budget && !(r = read_ctr(rs, 3, &nh)) && nh > 0)
+            r = run_heavy(rs, flags, rs->d_heavy, nh, 256, 0, 10, hwaves);
     }
     if (r) return r;
-    HIP_OK(hipEventRecord(rs->ev[1], idx->stream));
     int32_t nf = 0;
     if ((r = read_ctr(rs, 0, &nf))) return r;
-    if (nf > 0) {
-        // a first-tier failure may also be an output-arena overflow
-        unsigned long long top = 0;
-        HIP_OK(hipMemcpy(&top, rs->d_top, sizeof(top), hipMemcpyDeviceToHost));
-        if (top > rs->rec_cap && (r = grow_records(rs))) return r;
-        if ((r = run_overflow(rs, flags, 0, nf))) return r;
+    if (nf > 0 && (r = run_retries(rs, flags, 0, nf, t0, slots, budget, std::max(1, hwaves)))) return r;
+    HIP_OK(hipStreamSynchronize(idx->stream));
+    std::vector<unsigned long long> ab(rs->launches.size());
+    if (!ab.empty())
+        HIP_OK(hipMemcpy(ab.data(), rs->d_algo, ab.size() * sizeof(unsigned long long), hipMemcpyDeviceToHost));
+    rs->first_ms = rs->total_ms = 0;
+    for (size_t i = 0; i < rs->launches.size(); ++i) {
+        float ms = 0;
+        HIP_OK(hipEventElapsedTime(&ms, rs->lev[2 * i], rs->lev[2 * i + 1]));
+        rs->launches[i].ms = ms;
+        rs->launches[i].bytes = ab[i];
+        if (rs->launches[i].tier == 0) rs->first_ms += ms;
+        rs->total_ms += ms;
     }
-    HIP_OK(hipEventRecord(rs->ev[2], idx->stream));
-    HIP_OK(hipEventSynchronize(rs->ev[2]));
-    float ms1 = 0, ms2 = 0;
-    HIP_OK(hipEventElapsedTime(&ms1, rs->ev[0], rs->ev[1]));
-    HIP_OK(hipEventElapsedTime(&ms2, rs->ev[0], rs->ev[2]));
-    rs->first_ms = ms1;
-    rs->total_ms = ms2;
     return 0;
 }
 
@@ -1200,6 +2256,9 @@ int galn_hip_index_close(galn_hip_index *idx)
     hipSetDevice(idx->device);
     if (idx->stream) hipStreamSynchronize(idx->stream);
     hipFree(idx->d_bwt);
+    void *pool[] = {idx->d_pool_chk, idx->d_pool_link, idx->d_gnext, idx->d_gbump, idx->d_ghead};
+    for (void *p : pool)
+        if (p) hipFree(p);
     if (idx->stream) hipStreamDestroy(idx->stream);
     delete idx;
     return 0;
@@ -1213,12 +2272,14 @@ int galn_hip_readset_free(galn_hip_readset *rs)
     hipSetDevice(rs->idx->device);
     hipStreamSynchronize(rs->idx->stream);
     void *ptrs[] = {rs->d_query, rs->d_qoff, rs->d_qlen, rs->d_maxdiff, rs->d_batch, rs->d_nstacks, rs->d_bopts,
-                    rs->d_sorted, rs->d_out_n, rs->d_out_off, rs->d_pops, rs->d_trace, rs->d_rec, rs->d_top,
-                    rs->d_fail[0], rs->d_fail[1], rs->d_ctr, rs->d_scratch};
+                    rs->d_sorted, rs->d_out_n, rs->d_out_off, rs->d_pops, rs->d_trace, rs->d_bytes, rs->d_rec, rs->d_top,
+                    rs->d_fail[0], rs->d_fail[1], rs->d_ctr, rs->d_scratch, rs->d_algo, rs->d_heavy, rs->d_hscratch,
+                    rs->d_stats};
     for (void *p : ptrs)
         if (p) hipFree(p);
     for (auto &e : rs->ev)
         if (e) hipEventDestroy(e);
+    for (auto &e : rs->lev) hipEventDestroy(e);
     delete rs;
     return 0;
 }
@@ -1250,17 +2311,22 @@ int galn_hip_readset_upload(galn_hip_index *idx, const galn_hip_reads *reads, ga
     rs->h_batch.assign(reads->batch, reads->batch + n);
     rs->h_nstacks.assign(reads->n_stacks, reads->n_stacks + reads->n_batches);
     rs->h_bopts.assign(reads->bopts, reads->bopts + reads->n_batches);
+    rs->max_entries = 0;
+    for (const auto &o : rs->h_bopts) rs->max_entries = std::max(rs->max_entries, std::max(o.max_entries, 0));
     const size_t nn = (size_t)std::max(n, 1);
-    auto M = [&](void **p, size_t b) -> bool { return hipMalloc(p, std::max<size_t>(b, 16)) == hipSuccess; };
+    auto M = [&](void **p, size_t b) __attribute__((always_inline)) -> bool { return hipMalloc(p, std::max<size_t>(b, 16)) == hipSuccess; };
     rs->rec_cap = std::max<uint64_t>((uint64_t)n * 2, 1 << 16);
     bool ok = M((void **)&rs->d_query, qbytes + 16) && M((void **)&rs->d_qoff, nn * 8) && M((void **)&rs->d_qlen, nn * 4) &&
               M((void **)&rs->d_maxdiff, nn * 4) && M((void **)&rs->d_batch, nn * 4) &&
               M((void **)&rs->d_nstacks, reads->n_batches * 4) &&
               M((void **)&rs->d_bopts, reads->n_batches * sizeof(galn_search_opt_t)) && M((void **)&rs->d_sorted, nn * 4) &&
               M((void **)&rs->d_out_n, nn * 4) && M((void **)&rs->d_out_off, nn * 8) && M((void **)&rs->d_pops, nn * 8) &&
-              M((void **)&rs->d_trace, nn * 8) && M((void **)&rs->d_rec, rs->rec_cap * sizeof(galn_aln_rec_t)) &&
+              M((void **)&rs->d_trace, nn * 8) && M((void **)&rs->d_bytes, nn * 8) &&
+              M((void **)&rs->d_rec, rs->rec_cap * sizeof(galn_aln_rec_t)) &&
               M((void **)&rs->d_top, 8) && M((void **)&rs->d_fail[0], nn * 4) && M((void **)&rs->d_fail[1], nn * 4) &&
-              M((void **)&rs->d_ctr, 8 * sizeof(int32_t));
+              M((void **)&rs->d_ctr, 8 * sizeof(int32_t)) && M((void **)&rs->d_heavy, nn * 4) &&
+              M((void **)&rs->d_stats, 16 * 8) &&
+              M((void **)&rs->d_algo, sizeof(unsigned long long) * galn_hip_readset::MAX_LAUNCH);
     if (!ok) {
         galn_hip_readset_free(rs);
         return fail(GALN_HIP_E_NOMEM, "readset_upload: device allocation failed");
@@ -1319,6 +2385,7 @@ int galn_hip_readset_results(galn_hip_readset *rs, galn_hip_out *out)
     HIP_OK(hipMemcpyAsync(&top, rs->d_top, 8, hipMemcpyDeviceToHost, s));
     if (out->n_pops) HIP_OK(hipMemcpyAsync(out->n_pops, rs->d_pops, (size_t)n * 8, hipMemcpyDeviceToHost, s));
     if (out->trace) HIP_OK(hipMemcpyAsync(out->trace, rs->d_trace, (size_t)n * 8, hipMemcpyDeviceToHost, s));
+    if (out->occ_bytes) HIP_OK(hipMemcpyAsync(out->occ_bytes, rs->d_bytes, (size_t)n * 8, hipMemcpyDeviceToHost, s));
     HIP_OK(hipStreamSynchronize(s));
     uint64_t need = 0;
     for (int i = 0; i < n; ++i)
@@ -1350,6 +2417,25 @@ int galn_hip_readset_kernel_ms(const galn_hip_readset *rs, double *first_tier_ms
     if (first_tier_ms) *first_tier_ms = rs->first_ms;
     if (total_ms) *total_ms = rs->total_ms;
     return 0;
+}
+
+int galn_hip_readset_engine_stats(galn_hip_readset *rs, uint64_t out[8])
+{
+    if (!rs || !out) return fail(GALN_HIP_E_ARG, "readset_engine_stats: null argument");
+    HIP_OK(hipMemcpy(out, rs->d_stats, 8 * 8, hipMemcpyDeviceToHost));
+    return 0;
+}
+
+int galn_hip_readset_launches(const galn_hip_readset *rs, int cap, int32_t *tier, double *ms, uint64_t *algo_bytes)
+{
+    if (!rs) return fail(GALN_HIP_E_ARG, "readset_launches: null handle");
+    const int n = (int)rs->launches.size();
+    for (int i = 0; i < n && i < cap; ++i) {
+        if (tier) tier[i] = rs->launches[i].tier;
+        if (ms) ms[i] = rs->launches[i].ms;
+        if (algo_bytes) algo_bytes[i] = rs->launches[i].bytes;
+    }
+    return n;
 }
 
 static int search_once(galn_hip_index *idx, const galn_hip_reads *reads, galn_hip_out *out, int form, uint32_t flags,

@@ -594,6 +594,7 @@ extern "C" int galn_aln_main(int argc, char **argv)
     O.n_rec = 0;
     O.n_pops = pops.data();
     O.trace = A.trace ? trace.data() : nullptr;
+    O.occ_bytes = nullptr;
     const uint32_t flags = A.trace ? GALN_HIP_TRACE : 0u;
     galn_hip_readset *rs = nullptr;
     r = galn_hip_readset_upload(idx, &R, &rs);

@@ -75,12 +75,15 @@ def suffix_array(text: torch.Tensor) -> torch.Tensor:
     if n >= (1 << 31):
         raise ValueError("suffix_array: text of 2^31 symbols or more needs the two-key sort")
     rank = text.to(torch.int64)
+    mult = max(n, int(rank.max()) + 1) + 1  # second keys lie in [0, mult)
     h = 1
     while True:
+        if h > 2 * n:
+            raise RuntimeError("suffix_array: ranks failed to separate")
         second = torch.zeros_like(rank)
         if h < n:
             second[: n - h] = rank[h:] + 1  # 0 = past the end (smallest)
-        key = rank * (n + 1) + second
+        key = rank * mult + second
         sa = torch.argsort(key)
         skey = key[sa]
         del key, second

@@ -1,0 +1,412 @@
+"""ctypes mirror of the two C ABIs (include/galn_hip.h, include/galn_host.h).
+
+The search runs only through ``libgaln_hip.so``; this module raises if the
+library is missing. Structures mirror the headers field for field.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+LIB_DIR = os.path.join(PKG, "lib")
+HIP_LIB = os.path.join(LIB_DIR, "libgaln_hip.so")
+HOST_LIB = os.path.join(LIB_DIR, "libgaln_host.so")
+GALN_BIN = os.path.join(PKG, "bin", "galn")
+
+GALN_HIP_TRACE = 0x1
+GALN_HIP_NOFUSE = 0x2
+FORM_PLAIN, FORM_BATCHED = 0, 1
+E_RECCAP = -5
+
+HIP_SYMBOLS = (
+    "galn_hip_abi_version", "galn_hip_get_limits", "galn_hip_last_error", "galn_hip_index_open",
+    "galn_hip_index_close", "galn_hip_index_device", "galn_hip_search_plain", "galn_hip_search_batched",
+    "galn_hip_readset_upload", "galn_hip_readset_free", "galn_hip_readset_search", "galn_hip_readset_results",
+    "galn_hip_readset_kernel_ms", "galn_hip_readset_launches", "galn_hip_readset_engine_stats",
+    "galn_hip_gather_ceiling",
+)
+HOST_SYMBOLS = (
+    "galn_gap_opt_default", "galn_parse_aln_args", "galn_cal_maxdiff", "galn_batch_opt", "galn_read_maxdiff",
+    "galn_prepare_batches", "galn_load_bwt", "galn_free_bwt", "galn_read_fastq", "galn_readbuf_n",
+    "galn_readbuf_n_batches", "galn_readbuf_query", "galn_readbuf_qoff", "galn_readbuf_qlen", "galn_readbuf_batch",
+    "galn_readbuf_name", "galn_readbuf_free", "galn_sai_write", "galn_aln_main",
+)
+
+
+class BwtMeta(C.Structure):
+    _fields_ = [("primary", C.c_uint64), ("L2", C.c_uint64 * 5), ("seq_len", C.c_uint64), ("bwt_size", C.c_uint64)]
+
+
+class SearchOpt(C.Structure):
+    _fields_ = [(n, C.c_int32) for n in (
+        "s_mm", "s_gapo", "s_gape", "mode", "indel_end_skip", "max_del_occ", "max_entries", "max_diff",
+        "max_gapo", "max_gape", "max_seed_diff", "seed_len", "max_top2")]
+
+
+class AlnRec(C.Structure):
+    _fields_ = [("packed", C.c_uint64), ("k", C.c_uint64), ("l", C.c_uint64)]
+
+
+class Reads(C.Structure):
+    _fields_ = [("n_reads", C.c_int32), ("query", C.c_void_p), ("qoff", C.c_void_p), ("qlen", C.c_void_p),
+                ("max_diff", C.c_void_p), ("batch", C.c_void_p), ("n_batches", C.c_int32), ("bopts", C.c_void_p),
+                ("n_stacks", C.c_void_p), ("seed_len", C.c_int32)]
+
+
+class Out(C.Structure):
+    _fields_ = [("n_aln", C.c_void_p), ("rec_off", C.c_void_p), ("rec", C.c_void_p), ("rec_cap", C.c_uint64),
+                ("n_rec", C.c_uint64), ("n_pops", C.c_void_p), ("trace", C.c_void_p), ("occ_bytes", C.c_void_p)]
+
+
+class Limits(C.Structure):
+    _fields_ = [("max_read_len", C.c_int32), ("max_n_stacks", C.c_int32), ("max_diff", C.c_int32),
+                ("max_gapo_plus_gape", C.c_int32), ("max_seq_len", C.c_uint64)]
+
+
+class Tune(C.Structure):
+    _fields_ = [(n, C.c_int32) for n in ("block_threads", "waves_per_cu", "chunks_per_lane", "rec_per_lane",
+                                         "sort_by_cost", "heavy_pops")]
+
+
+class GapOpt(C.Structure):
+    _fields_ = [("s_mm", C.c_int32), ("s_gapo", C.c_int32), ("s_gape", C.c_int32), ("mode", C.c_int32),
+                ("indel_end_skip", C.c_int32), ("max_del_occ", C.c_int32), ("max_entries", C.c_int32),
+                ("fnr", C.c_float), ("max_diff", C.c_int32), ("max_gapo", C.c_int32), ("max_gape", C.c_int32),
+                ("max_seed_diff", C.c_int32), ("seed_len", C.c_int32), ("n_threads", C.c_int32),
+                ("max_top2", C.c_int32), ("trim_qual", C.c_int32)]
+
+
+class AlnArgs(C.Structure):
+    _fields_ = [("opt", GapOpt), ("prefix", C.c_char_p), ("reads", C.c_char_p), ("out", C.c_char_p),
+                ("device", C.c_int), ("form", C.c_int), ("verbose", C.c_int), ("trace", C.c_int)]
+
+
+class GalnError(RuntimeError):
+    pass
+
+
+_hip = None
+_host = None
+
+
+def hip_lib():
+    """libgaln_hip.so; raises if it is not built (there is no fallback search)."""
+    global _hip
+    if _hip is None:
+        if not os.path.exists(HIP_LIB):
+            raise GalnError(f"{HIP_LIB} is missing: build it with `python -m galn_amd.build`")
+        L = C.CDLL(HIP_LIB)
+        P = C.c_void_p
+        L.galn_hip_last_error.restype = C.c_char_p
+        L.galn_hip_index_open.argtypes = [P, C.POINTER(BwtMeta), C.c_int, C.POINTER(P)]
+        L.galn_hip_index_close.argtypes = [P]
+        L.galn_hip_readset_upload.argtypes = [P, C.POINTER(Reads), C.POINTER(P)]
+        L.galn_hip_readset_free.argtypes = [P]
+        L.galn_hip_readset_search.argtypes = [P, P, C.c_int, C.c_uint32, C.POINTER(Tune)]
+        L.galn_hip_readset_results.argtypes = [P, C.POINTER(Out)]
+        L.galn_hip_readset_kernel_ms.argtypes = [P, C.POINTER(C.c_double), C.POINTER(C.c_double)]
+        L.galn_hip_readset_launches.argtypes = [P, C.c_int, P, P, P]
+        L.galn_hip_readset_engine_stats.argtypes = [P, P]
+        L.galn_hip_search_plain.argtypes = [P, C.POINTER(Reads), C.POINTER(Out), C.c_uint32]
+        L.galn_hip_search_batched.argtypes = [P, C.POINTER(Reads), C.POINTER(Out), C.c_uint32, C.POINTER(Tune)]
+        L.galn_hip_get_limits.argtypes = [C.POINTER(Limits)]
+        L.galn_hip_gather_ceiling.argtypes = [C.c_int, C.c_uint64, C.c_int, C.c_int, C.c_int,
+                                              C.POINTER(C.c_double), C.POINTER(C.c_double)]
+        _hip = L
+    return _hip
+
+
+def host_lib():
+    global _host
+    if _host is None:
+        if not os.path.exists(HOST_LIB):
+            raise GalnError(f"{HOST_LIB} is missing: build it with `python -m galn_amd.build`")
+        L = C.CDLL(HOST_LIB)
+        P = C.c_void_p
+        L.galn_cal_maxdiff.restype = C.c_int
+        L.galn_cal_maxdiff.argtypes = [C.c_int, C.c_double, C.c_double]
+        L.galn_gap_opt_default.argtypes = [C.POINTER(GapOpt)]
+        L.galn_parse_aln_args.argtypes = [C.c_int, C.POINTER(C.c_char_p), C.POINTER(AlnArgs)]
+        L.galn_batch_opt.argtypes = [C.POINTER(GapOpt), C.c_int, C.POINTER(SearchOpt)]
+        L.galn_prepare_batches.argtypes = [C.POINTER(GapOpt), C.c_int, P, P, C.c_int, P, P, P]
+        L.galn_load_bwt.restype = P
+        L.galn_load_bwt.argtypes = [C.c_char_p, C.POINTER(BwtMeta)]
+        L.galn_free_bwt.argtypes = [P]
+        L.galn_read_fastq.restype = P
+        L.galn_read_fastq.argtypes = [C.c_char_p]
+        for f in ("galn_readbuf_n", "galn_readbuf_n_batches"):
+            getattr(L, f).restype = C.c_int32
+            getattr(L, f).argtypes = [P]
+        for f in ("galn_readbuf_query", "galn_readbuf_qoff", "galn_readbuf_qlen", "galn_readbuf_batch"):
+            getattr(L, f).restype = P
+            getattr(L, f).argtypes = [P]
+        L.galn_readbuf_name.restype = C.c_char_p
+        L.galn_readbuf_name.argtypes = [P, C.c_int32]
+        L.galn_readbuf_free.argtypes = [P]
+        L.galn_aln_main.argtypes = [C.c_int, C.POINTER(C.c_char_p)]
+        _host = L
+    return _host
+
+
+def last_error() -> str:
+    return hip_lib().galn_hip_last_error().decode(errors="replace")
+
+
+def _p(a: np.ndarray):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+def gather_ceiling(ws_bytes: int, line_bytes: int = 64, waves_per_cu: int = 16, iters: int = 2000,
+                   device: int = 0) -> tuple[float, float]:
+    """(line gathers per second, ms) of the pointer-chase microbenchmark."""
+    lps, ms = C.c_double(), C.c_double()
+    rc = hip_lib().galn_hip_gather_ceiling(device, int(ws_bytes), int(line_bytes), int(waves_per_cu), int(iters),
+                                           C.byref(lps), C.byref(ms))
+    if rc != 0:
+        raise GalnError(f"gather_ceiling failed ({rc})")
+    return lps.value, ms.value
+
+
+def limits() -> dict:
+    lim = Limits()
+    hip_lib().galn_hip_get_limits(C.byref(lim))
+    return {f: getattr(lim, f) for f, _ in Limits._fields_}
+
+
+# --------------------------------------------------------------------------
+# host side
+
+
+def parse_aln_args(args: list[str]) -> tuple[int, AlnArgs]:
+    """galn_parse_aln_args over a Python list (without "galn aln")."""
+    arr = (C.c_char_p * max(1, len(args)))(*[a.encode() for a in args])
+    out = AlnArgs()
+    rc = host_lib().galn_parse_aln_args(len(args), arr, C.byref(out))
+    out._keep = arr  # the parsed strings point into arr
+    return rc, out
+
+
+def gap_opt_from_args(opt_args: list[str]) -> GapOpt:
+    rc, a = parse_aln_args(list(opt_args) + ["prefix", "reads"])
+    if rc != 0:
+        raise GalnError(f"bad aln options: {opt_args}")
+    return a.opt
+
+
+def gap_opt_bytes(o: GapOpt) -> bytes:
+    return bytes(memoryview(o))
+
+
+def prepare_batches(o: GapOpt, qlen: np.ndarray, batch: np.ndarray):
+    """Per-batch SearchOpt array, n_stacks and per-read max_diff (galn_prepare_batches)."""
+    n = int(qlen.size)
+    ql = np.ascontiguousarray(qlen, dtype=np.int32)
+    bi = np.ascontiguousarray(batch, dtype=np.int32)
+    nb = int(bi[-1]) + 1 if n else 1
+    bopts = (SearchOpt * nb)()
+    nst = np.zeros(nb, np.int32)
+    md = np.zeros(max(n, 1), np.int32)
+    got = host_lib().galn_prepare_batches(C.byref(o), n, _p(ql), _p(bi), nb, C.cast(bopts, C.c_void_p), _p(nst), _p(md))
+    if got != nb:
+        raise GalnError(f"prepare_batches: batch ids not 0..{nb - 1} in order")
+    return bopts, nst, md[:n]
+
+
+def load_bwt(path: str) -> tuple[BwtMeta, np.ndarray]:
+    m = BwtMeta()
+    L = host_lib()
+    p = L.galn_load_bwt(path.encode(), C.byref(m))
+    if not p:
+        raise GalnError(f"cannot load {path}")
+    arr = np.ctypeslib.as_array(C.cast(p, C.POINTER(C.c_uint32)), shape=(int(m.bwt_size),)).copy()
+    L.galn_free_bwt(p)
+    return m, arr
+
+
+def meta_from_dict(d: dict) -> BwtMeta:
+    m = BwtMeta()
+    m.primary = int(d["primary"])
+    for i in range(5):
+        m.L2[i] = int(d["L2"][i])
+    m.seq_len = int(d["seq_len"])
+    m.bwt_size = int(d["bwt_size"])
+    return m
+
+
+def read_fastq(path: str) -> dict:
+    """galn_read_fastq: queries, offsets, lengths, batch ids and names."""
+    L = host_lib()
+    h = L.galn_read_fastq(path.encode())
+    if not h:
+        raise GalnError(f"cannot read {path}")
+    try:
+        n = L.galn_readbuf_n(h)
+        nb = L.galn_readbuf_n_batches(h)
+
+        def arr(fn, ctype, count):
+            if count == 0:
+                return np.zeros(0, dtype=np.ctypeslib.as_ctypes_type(ctype) and ctype)
+            return np.ctypeslib.as_array(C.cast(getattr(L, fn)(h), C.POINTER(np.ctypeslib.as_ctypes_type(ctype))),
+                                         shape=(count,)).copy()
+
+        qlen = arr("galn_readbuf_qlen", np.int32, n)
+        qoff = arr("galn_readbuf_qoff", np.uint64, n)
+        total = int(qoff[-1] + qlen[-1]) if n else 0
+        query = arr("galn_readbuf_query", np.uint8, total)
+        batch = arr("galn_readbuf_batch", np.int32, n)
+        names = [L.galn_readbuf_name(h, i).decode() for i in range(n)] if n <= 1_000_000 else None
+    finally:
+        L.galn_readbuf_free(h)
+    return {"n": n, "n_batches": nb, "query": query, "qoff": qoff, "qlen": qlen, "batch": batch, "names": names}
+
+
+# --------------------------------------------------------------------------
+# search
+
+
+class Index:
+    """A .bwt body resident in one device's HBM."""
+
+    def __init__(self, words: np.ndarray, meta: BwtMeta | dict, device: int = 0):
+        self.meta = meta if isinstance(meta, BwtMeta) else meta_from_dict(meta)
+        self._words = np.ascontiguousarray(words, dtype=np.uint32)
+        h = C.c_void_p()
+        rc = hip_lib().galn_hip_index_open(_p(self._words), C.byref(self.meta), int(device), C.byref(h))
+        if rc != 0:
+            raise GalnError(f"index_open failed ({rc}): {last_error()}")
+        self.h = h
+        self._words = None
+
+    def close(self):
+        if self.h:
+            hip_lib().galn_hip_index_close(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Results:
+    def __init__(self, n_aln, rec_off, rec, pops, trace, occ_bytes):
+        self.n_aln, self.rec_off, self.rec = n_aln, rec_off, rec
+        self.pops, self.trace, self.occ_bytes = pops, trace, occ_bytes
+
+    def records(self, i: int) -> np.ndarray:
+        n = max(int(self.n_aln[i]), 0)
+        o = int(self.rec_off[i])
+        return self.rec[o:o + n]
+
+
+class ReadSet:
+    """Reads resident on the device (galn_hip_readset_*)."""
+
+    def __init__(self, index: Index, query, qoff, qlen, max_diff, batch, bopts, n_stacks, seed_len: int):
+        self.index = index
+        self._arrays = [np.ascontiguousarray(query, dtype=np.uint8), np.ascontiguousarray(qoff, dtype=np.uint64),
+                        np.ascontiguousarray(qlen, dtype=np.int32), np.ascontiguousarray(max_diff, dtype=np.int32),
+                        np.ascontiguousarray(batch, dtype=np.int32), np.ascontiguousarray(n_stacks, dtype=np.int32)]
+        q, qo, ql, md, bi, ns = self._arrays
+        self.n = int(ql.size)
+        R = Reads()
+        R.n_reads = self.n
+        R.query, R.qoff, R.qlen, R.max_diff, R.batch = _p(q), _p(qo), _p(ql), _p(md), _p(bi)
+        R.n_batches = int(ns.size)
+        R.bopts = C.cast(bopts, C.c_void_p)
+        R.n_stacks = _p(ns)
+        R.seed_len = int(seed_len)
+        self._bopts = bopts
+        h = C.c_void_p()
+        rc = hip_lib().galn_hip_readset_upload(index.h, C.byref(R), C.byref(h))
+        if rc != 0:
+            raise GalnError(f"readset_upload failed ({rc}): {last_error()}")
+        self.h = h
+        self._arrays = None
+
+    def search(self, form: int = FORM_BATCHED, flags: int = 0, tune: Tune | None = None) -> None:
+        rc = hip_lib().galn_hip_readset_search(self.index.h, self.h, int(form), int(flags),
+                                               C.byref(tune) if tune is not None else None)
+        if rc != 0:
+            raise GalnError(f"readset_search failed ({rc}): {last_error()}")
+
+    def launches(self) -> list[dict]:
+        """[{tier, ms, algo_bytes}] for every kernel launch of the last search."""
+        L = hip_lib()
+        n = L.galn_hip_readset_launches(self.h, 0, None, None, None)
+        tier = np.zeros(max(n, 1), np.int32)
+        ms = np.zeros(max(n, 1), np.float64)
+        ab = np.zeros(max(n, 1), np.uint64)
+        L.galn_hip_readset_launches(self.h, n, _p(tier), _p(ms), _p(ab))
+        return [{"tier": int(tier[i]), "ms": float(ms[i]), "algo_bytes": int(ab[i])} for i in range(n)]
+
+    def engine_stats(self) -> dict:
+        """Counters of the wave-per-read engine in the last search."""
+        v = np.zeros(8, np.uint64)
+        hip_lib().galn_hip_readset_engine_stats(self.h, _p(v))
+        keys = ("rounds", "chains", "chains_committed", "steps_committed", "critical_steps", "spec_ticks",
+                "commit_ticks")
+        return {k: int(v[i]) for i, k in enumerate(keys)}
+
+    def kernel_ms(self) -> tuple[float, float]:
+        a, b = C.c_double(), C.c_double()
+        hip_lib().galn_hip_readset_kernel_ms(self.h, C.byref(a), C.byref(b))
+        return a.value, b.value
+
+    def results(self, want_pops=True, want_trace=False, want_bytes=False, rec_cap: int | None = None) -> Results:
+        n = self.n
+        n_aln = np.zeros(max(n, 1), np.int32)
+        rec_off = np.zeros(max(n, 1), np.uint64)
+        pops = np.zeros(max(n, 1), np.uint64) if want_pops else None
+        trace = np.zeros(max(n, 1), np.uint64) if want_trace else None
+        ob = np.zeros(max(n, 1), np.uint64) if want_bytes else None
+        cap = rec_cap if rec_cap is not None else max(2 * n, 1)
+        while True:
+            rec = np.zeros((max(cap, 1), 3), np.uint64)
+            O = Out()
+            O.n_aln, O.rec_off, O.rec, O.rec_cap = _p(n_aln), _p(rec_off), _p(rec), cap
+            O.n_pops = _p(pops) if pops is not None else None
+            O.trace = _p(trace) if trace is not None else None
+            O.occ_bytes = _p(ob) if ob is not None else None
+            rc = hip_lib().galn_hip_readset_results(self.h, C.byref(O))
+            if rc == E_RECCAP:
+                cap = int(O.n_rec)
+                continue
+            if rc != 0:
+                raise GalnError(f"readset_results failed ({rc}): {last_error()}")
+            return Results(n_aln[:n], rec_off[:n], rec[:int(O.n_rec)],
+                           pops[:n] if pops is not None else None, trace[:n] if trace is not None else None,
+                           ob[:n] if ob is not None else None)
+
+    def close(self):
+        if self.h:
+            hip_lib().galn_hip_readset_free(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def sai_bytes(opt: GapOpt, n_aln: np.ndarray, rec_off: np.ndarray, rec: np.ndarray) -> bytes:
+    """The .sai byte stream (the same layout galn_sai_write produces)."""
+    parts = [b"SAI\x01", gap_opt_bytes(opt)]
+    na = np.where(n_aln > 0, n_aln, 0).astype(np.int32)
+    for i in range(na.size):
+        parts.append(int(na[i]).to_bytes(4, "little", signed=True))
+        if na[i]:
+            o = int(rec_off[i])
+            parts.append(np.ascontiguousarray(rec[o:o + int(na[i])], dtype=np.uint64).tobytes())
+    return b"".join(parts)

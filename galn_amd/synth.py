@@ -141,60 +141,92 @@ def adna_lengths(rng, n: int, median: float = 45.0, sigma: float = 0.45,
     return np.clip(np.rint(raw), lo, hi).astype(np.int64)
 
 
+def _adna_chunk(args):
+    """One chunk of adna_reads; a pure function of (seed, chunk index)."""
+    seed, c, ref, m, lens, random_frac, sub_rate, deam, hi = args
+    rng = np.random.default_rng([seed, c])
+    L = ref.size
+    win = np.lib.stride_tricks.sliding_window_view(ref, hi)
+    # window [s, s + hi): forward reads start at s, reverse reads end at s + hi
+    start = (rng.random(m) * (L - hi)).astype(np.int64)
+    frag = win[start].copy()
+    rev = rng.random(m) < 0.5
+    if rev.any():
+        # revcomp of the window's last len bases = the first len bases of revcomp(window)
+        sub = frag[rev]
+        frag[rev] = _COMP[sub[:, ::-1]]
+    unrel = rng.random(m) < random_frac
+    if unrel.any():
+        frag[unrel] = random_seq(rng, int(unrel.sum()) * hi, 0.41).reshape(-1, hi)
+    # deamination: 5' C->T with probability deam * 0.5^i at position i, mirrored G->A at the 3' end
+    # (positions beyond 16 from either end are left alone: probability < 5e-6)
+    nd = min(16, hi)
+    pdec = (deam * 0.5 ** np.arange(nd)).astype(np.float32)
+    u5 = rng.random((m, nd), dtype=np.float32) < pdec[None, :]
+    head = frag[:, :nd]
+    ct = u5 & (head == ord("C")) & (np.arange(nd)[None, :] < lens[:, None])
+    head[ct] = ord("T")
+    u3 = rng.random((m, nd), dtype=np.float32) < pdec[None, :]
+    pos3 = lens[:, None] - 1 - np.arange(nd)[None, :]  # j-th base from the 3' end
+    ok3 = pos3 >= 0
+    rows = np.nonzero(u3 & ok3)[0]
+    cols = pos3[u3 & ok3]
+    isg = frag[rows, cols] == ord("G")
+    frag[rows[isg], cols[isg]] = ord("A")
+    # substitutions: sparse events over the read bodies
+    total = int(lens.sum())
+    ne = rng.binomial(total, sub_rate)
+    flat = rng.integers(0, total, size=ne)
+    offs = np.zeros(m + 1, dtype=np.int64)
+    np.cumsum(lens, out=offs[1:])
+    r = np.searchsorted(offs, flat, side="right") - 1
+    cpos = flat - offs[r]
+    cur = _NT4[frag[r, cpos]]
+    keep_n = cur > 3
+    newb = ACGT[(np.where(keep_n, 0, cur) + rng.integers(1, 4, size=ne)) & 3]
+    frag[r, cpos] = np.where(keep_n, frag[r, cpos], newb)
+    frag[np.arange(hi)[None, :] >= lens[:, None]] = 0
+    return frag
+
+
 def adna_reads(seed: int, ref: np.ndarray, n: int, random_frac: float = 0.05, sub_rate: float = 0.005,
                deam: float = 0.30, lo: int = 30, hi: int = 150, median: float = 45.0,
-               chunk: int = 1 << 20) -> tuple[np.ndarray, np.ndarray]:
+               chunk: int = 1 << 18, workers: int = 1) -> tuple[np.ndarray, np.ndarray]:
     """aDNA-like reads: log-normal lengths, deamination at both ends, substitutions,
-    and a share of reads drawn from an unrelated random sequence."""
+    and a share of reads drawn from an unrelated random sequence. Chunks are
+    independent (seeded by (seed, chunk)), so ``workers`` only changes the speed."""
     rng = np.random.default_rng(seed)
-    out = np.zeros((n, hi), dtype=np.uint8)
     lens_all = adna_lengths(rng, n, median=median, lo=lo, hi=hi)
-    decay = deam * 0.5 ** np.arange(hi)
-    for s in range(0, n, chunk):
-        e = min(n, s + chunk)
-        m = e - s
-        lens = lens_all[s:e]
-        frag, _ = _sample_fragments(rng, ref, lens, hi, 0)
-        strand = rng.random(m) < 0.5
-        frag[strand] = revcomp_rows(frag[strand], lens[strand])
-        unrel = rng.random(m) < random_frac
-        if unrel.any():
-            frag[unrel] = random_seq(rng, int(unrel.sum()) * hi, 0.41).reshape(-1, hi)
-        cols = np.arange(hi)[None, :]
-        inread = cols < lens[:, None]
-        # 5' C->T with probability deam * 0.5^i at position i
-        ct = (frag == ord("C")) & (rng.random((m, hi)) < decay[None, :]) & inread
-        frag[ct] = ord("T")
-        # 3' G->A mirrored: position j from the 3' end
-        from3 = lens[:, None] - 1 - cols
-        p3 = np.where(inread, deam * 0.5 ** np.clip(from3, 0, 60), 0.0)
-        ga = (frag == ord("G")) & (rng.random((m, hi)) < p3)
-        frag[ga] = ord("A")
-        sub = (rng.random((m, hi)) < sub_rate) & inread
-        rows, cc = np.nonzero(sub)
-        cur = _NT4[frag[rows, cc]]
-        keep_n = cur > 3
-        cur = np.where(keep_n, 0, cur)
-        newb = ACGT[(cur + rng.integers(1, 4, size=rows.size)) & 3]
-        frag[rows, cc] = np.where(keep_n, frag[rows, cc], newb)
-        frag[~inread] = 0
-        out[s:e] = frag
+    out = np.zeros((n, hi), dtype=np.uint8)
+    jobs = [(seed, c, ref, min(chunk, n - s), lens_all[s:s + chunk], random_frac, sub_rate, deam, hi)
+            for c, s in enumerate(range(0, n, chunk))]
+    if workers > 1 and len(jobs) > 1:
+        from concurrent.futures import ThreadPoolExecutor
+        with ThreadPoolExecutor(workers) as ex:
+            for (c, s), frag in zip(enumerate(range(0, n, chunk)), ex.map(_adna_chunk, jobs)):
+                out[s:s + frag.shape[0]] = frag
+    else:
+        for (c, s), job in zip(enumerate(range(0, n, chunk)), jobs):
+            out[s:s + job[3]] = _adna_chunk(job)
     return out, lens_all
 
 
 def encode_queries(reads: np.ndarray, lens: np.ndarray) -> tuple[np.ndarray, np.ndarray]:
     """Search queries: q[j] = complement(nt4(read[len-1-j])), N -> 4 (reference
-    src/host_io.c:178-182). Returns (concatenated bytes, u64 offsets)."""
+    src/host_io.c:178-182). Returns (bytes, u64 offsets). The bytes are the
+    reversed concatenation of the reads, so read i's query starts at
+    total - (start of read i) - len(i)."""
     n, w = reads.shape
     lens = np.asarray(lens, dtype=np.int64)
-    idx = lens[:, None] - 1 - np.arange(w)[None, :]
-    valid = idx >= 0
-    b = _NT4[reads[np.nonzero(valid)[0], idx[valid]]]
+    mask = np.arange(w)[None, :] < lens[:, None]
+    cat = reads[mask]
+    b = _NT4[cat[::-1]]
     q = np.where(b > 3, 4, 3 - b).astype(np.uint8)
-    off = np.zeros(n, dtype=np.uint64)
+    starts = np.zeros(n, dtype=np.int64)
     if n:
-        off[1:] = np.cumsum(lens[:-1]).astype(np.uint64)
-    return q, off
+        starts[1:] = np.cumsum(lens[:-1])
+    qoff = (int(lens.sum()) - starts - lens).astype(np.uint64)
+    return q, qoff
 
 
 def write_fasta(path: str, contigs: list[tuple[str, np.ndarray]], width: int = 60) -> None:
@@ -217,3 +249,101 @@ def write_fastq(path: str, reads: np.ndarray, lens: np.ndarray, prefix: str = "r
                 fh.write(b"".join(parts))
                 parts = []
         fh.write(b"".join(parts))
+
+
+def human_like_reference(seed: int, length: int, n_contigs: int = 24, gc: float = 0.41,
+                         repeat_frac: float = 0.45, n_families: int = 50, tandem_frac: float = 0.03
+                         ) -> list[tuple[str, np.ndarray]]:
+    """Contigs with interspersed repeats from seed families (2-20% divergence) and
+    tandem repeats of period 2-200."""
+    rng = np.random.default_rng(seed)
+    s = random_seq(rng, length, gc)
+    fams = [random_seq(rng, int(rng.integers(300, 6001)), gc) for _ in range(n_families)]
+    target, done = int(length * repeat_frac), 0
+    while done < target:
+        f = fams[int(rng.integers(0, n_families))]
+        cut = int(rng.integers(0, max(1, f.size // 3)))
+        piece = f[cut:]
+        if piece.size >= length:
+            break
+        dst = int(rng.integers(0, length - piece.size))
+        s[dst:dst + piece.size] = mutate_subs(rng, piece, float(rng.uniform(0.02, 0.20)))
+        done += piece.size
+    target, done = int(length * tandem_frac), 0
+    while done < target:
+        period = int(rng.integers(2, 201))
+        unit = random_seq(rng, period, gc)
+        copies = int(rng.integers(3, 40))
+        rep = np.tile(unit, copies)
+        if rep.size >= length:
+            continue
+        dst = int(rng.integers(0, length - rep.size))
+        s[dst:dst + rep.size] = mutate_subs(rng, rep, 0.01)
+        done += rep.size
+    cuts = np.sort(rng.choice(np.arange(1, length), size=n_contigs - 1, replace=False))
+    parts = np.split(s, cuts)
+    return [(f"chr{i + 1}", p) for i, p in enumerate(parts)]
+
+
+def _indel_copy(rng, base: np.ndarray, sub_rate: float, indel_rate: float) -> np.ndarray:
+    """A diverged copy: substitutions, then 1-3 bp insertions/deletions at indel_rate per base."""
+    s = mutate_subs(rng, base, sub_rate)
+    ev = np.flatnonzero(rng.random(s.size) < indel_rate)
+    if ev.size == 0:
+        return s
+    out, last = [], 0
+    for p in ev:
+        out.append(s[last:p])
+        k = int(rng.integers(1, 4))
+        if rng.random() < 0.5:
+            out.append(ACGT[rng.integers(0, 4, size=k)])
+            last = p
+        else:
+            last = min(s.size, p + k)
+    out.append(s[last:])
+    return np.concatenate(out)
+
+
+def diverged_copies_reference(seed: int, base_len: int, copies: int = 8) -> list[tuple[str, np.ndarray]]:
+    """A random base genome and `copies` copies of it, each with 1-3% substitutions and 0.2% indels."""
+    rng = np.random.default_rng(seed)
+    base = random_seq(rng, base_len, 0.5)
+    return [(f"copy{i + 1}", _indel_copy(rng, base, float(rng.uniform(0.01, 0.03)), 0.002)) for i in range(copies)]
+
+
+def paired_reads(seed: int, ref: np.ndarray, n_pairs: int, length: int = 150, insert_mean: float = 350.0,
+                 insert_sd: float = 50.0, err: float = 0.02) -> tuple[np.ndarray, np.ndarray, np.ndarray, np.ndarray]:
+    """Paired-end reads: fragment length ~ N(insert_mean, insert_sd); mate 1 from the fragment's
+    start, mate 2 the reverse complement of its end; substitutions at rate `err`."""
+    rng = np.random.default_rng(seed)
+    ins = np.clip(np.rint(rng.normal(insert_mean, insert_sd, n_pairs)), length, 4 * insert_mean).astype(np.int64)
+    start = (rng.random(n_pairs) * (ref.size - ins - 1)).astype(np.int64)
+    cols = np.arange(length)[None, :]
+    m1 = ref[start[:, None] + cols]
+    m2 = _COMP[ref[(start + ins - 1)[:, None] - cols]]
+    flip = rng.random(n_pairs) < 0.5
+    m1[flip], m2[flip] = m2[flip].copy(), m1[flip].copy()
+    lens = np.full(n_pairs, length, dtype=np.int64)
+    out = []
+    for m in (m1, m2):
+        mm = m.copy()
+        sub = rng.random(mm.shape) < err
+        rows, cc = np.nonzero(sub)
+        cur = _NT4[mm[rows, cc]]
+        cur = np.where(cur > 3, 0, cur)
+        mm[rows, cc] = ACGT[(cur + rng.integers(1, 4, size=rows.size)) & 3]
+        out.append(mm)
+    return out[0], lens, out[1], lens.copy()
+
+
+def uniform_length_reads(seed: int, ref: np.ndarray, n: int, lo: int = 30, hi: int = 150, sub_rate: float = 0.005,
+                         deam: float = 0.30, random_frac: float = 0.0) -> tuple[np.ndarray, np.ndarray]:
+    """Reads of uniform length in [lo, hi], both strands, deamination and substitutions."""
+    rng = np.random.default_rng(seed)
+    lens = rng.integers(lo, hi + 1, size=n).astype(np.int64)
+    out = np.zeros((n, hi), dtype=np.uint8)
+    chunk = 1 << 18
+    for c, s in enumerate(range(0, n, chunk)):
+        m = min(chunk, n - s)
+        out[s:s + m] = _adna_chunk((seed, c, ref, m, lens[s:s + m], random_frac, sub_rate, deam, hi))
+    return out, lens

@@ -1,0 +1,102 @@
+"""The aln workloads, built from seeds at any scale (tests use scaled copies).
+
+A workload is a reference (ASCII bases), its .bwt (written under ``workdir``),
+reads (padded ASCII matrix + lengths), the aln option list and the derived
+search inputs: queries, offsets, bwa batch ids, per-batch options, n_stacks and
+per-read max_diff (computed by the product's host library).
+"""
+from __future__ import annotations
+
+import os
+import time
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import index as ix
+from . import native as nv
+from . import synth
+
+ADNA_OPTS = ["-l", "1024", "-n", "0.01", "-o", "2"]
+DEFAULT_OPTS: list[str] = []
+
+
+@dataclass
+class Workload:
+    name: str
+    prefix: str
+    meta: dict
+    words: np.ndarray
+    reads: np.ndarray
+    lens: np.ndarray
+    opts: list[str]
+    query: np.ndarray = field(default=None)
+    qoff: np.ndarray = field(default=None)
+    batch: np.ndarray = field(default=None)
+    gap_opt: object = field(default=None)
+    bopts: object = field(default=None)
+    nstacks: np.ndarray = field(default=None)
+    maxdiff: np.ndarray = field(default=None)
+    timings: dict = field(default_factory=dict)
+
+    def prepare(self) -> "Workload":
+        t = time.time()
+        self.query, self.qoff = synth.encode_queries(self.reads, self.lens)
+        n = int(self.lens.size)
+        self.batch = (np.arange(n, dtype=np.int64) // 0x40000).astype(np.int32)
+        self.gap_opt = nv.gap_opt_from_args(self.opts)
+        self.bopts, self.nstacks, self.maxdiff = nv.prepare_batches(self.gap_opt, self.lens.astype(np.int32), self.batch)
+        self.timings["prepare_s"] = time.time() - t
+        return self
+
+    @property
+    def n(self) -> int:
+        return int(self.lens.size)
+
+    @property
+    def seed_len(self) -> int:
+        return int(self.gap_opt.seed_len)
+
+    def readset(self, index: "nv.Index") -> "nv.ReadSet":
+        return nv.ReadSet(index, self.query, self.qoff, self.lens.astype(np.int32), self.maxdiff, self.batch,
+                          self.bopts, self.nstacks, self.seed_len)
+
+    def fastq(self, path: str, start: int = 0, count: int | None = None) -> str:
+        end = self.n if count is None else min(self.n, start + count)
+        synth.write_fastq(path, self.reads[start:end], self.lens[start:end])
+        return path
+
+
+def _index(ref: np.ndarray, prefix: str, device: str) -> tuple[dict, np.ndarray]:
+    ix.build_index(ref.tobytes(), prefix, device=device)
+    return ix.read_bwt(prefix + ".bwt")
+
+
+def plain_small(workdir: str, ref_bp: int = 5_000_000, n_reads: int = 100_000, read_len: int = 100,
+                seeds: tuple[int, int] = (101, 102), device: str = "cpu") -> Workload:
+    """i.i.d. reference, fixed-length reads, default aln options."""
+    os.makedirs(workdir, exist_ok=True)
+    t = time.time()
+    ref = synth.plain_reference(seeds[0], ref_bp)
+    prefix = os.path.join(workdir, f"plain_{ref_bp}_{seeds[0]}")
+    meta, words = _index(ref, prefix, device)
+    t_idx = time.time() - t
+    reads, lens = synth.plain_reads(seeds[1], ref, n_reads, length=read_len)
+    w = Workload("plain_small", prefix, meta, words, reads, lens, list(DEFAULT_OPTS))
+    w.timings.update(index_s=t_idx, reads_s=time.time() - t - t_idx)
+    return w.prepare()
+
+
+def adna_mixed(workdir: str, ref_bp: int = 100_000_000, n_reads: int = 10_000_000,
+               seeds: tuple[int, int] = (201, 202), device: str = "cpu", workers: int = 8) -> Workload:
+    """Duplicated/N-run 41% GC reference, aDNA-like mixed-length reads, aln -l 1024 -n 0.01 -o 2."""
+    os.makedirs(workdir, exist_ok=True)
+    t = time.time()
+    ref = synth.adna_reference(seeds[0], ref_bp)
+    prefix = os.path.join(workdir, f"adna_{ref_bp}_{seeds[0]}")
+    meta, words = _index(ref, prefix, device)
+    t_idx = time.time() - t
+    reads, lens = synth.adna_reads(seeds[1], ref, n_reads, workers=workers)
+    w = Workload("adna_mixed", prefix, meta, words, reads, lens, list(ADNA_OPTS))
+    w.timings.update(index_s=t_idx, reads_s=time.time() - t - t_idx)
+    return w.prepare()

@@ -98,13 +98,15 @@ typedef struct {
     uint64_t n_rec;            /* out: records written (or needed, with GALN_HIP_E_RECCAP) */
     uint64_t *n_pops;          /* optional [n_reads]: popped entries */
     uint64_t *trace;           /* optional [n_reads]: trace hash (needs GALN_HIP_TRACE) */
+    uint64_t *occ_bytes;       /* optional [n_reads]: algorithmic index bytes, 64 per distinct
+                                  128-symbol block per occurrence query the search performs */
 } galn_hip_out;
 
 /* Kernel limits; reads beyond them get GALN_READ_ERR_UNSUP. */
 typedef struct {
     int32_t max_read_len;      /* 1023 */
     int32_t max_n_stacks;      /* 256 */
-    int32_t max_diff;          /* 127 */
+    int32_t max_diff;          /* 126 (n_mm may reach max_diff + 1 in 7 bits) */
     int32_t max_gapo_plus_gape;/* 255 */
     uint64_t max_seq_len;      /* 2^38 - 1 */
 } galn_hip_limits_t;
@@ -116,6 +118,8 @@ typedef struct {
     int32_t chunks_per_lane;   /* stack chunks (16 entries each) per lane in the first tier */
     int32_t rec_per_lane;      /* record slots per lane in the first tier */
     int32_t sort_by_cost;      /* 1: claim reads longest-first */
+    int32_t heavy_pops;        /* pops after which a read moves to the wave-per-read engine
+                                  (0: default, -1: never) */
 } galn_hip_tune_t;
 
 typedef struct galn_hip_index galn_hip_index;
@@ -153,6 +157,21 @@ int galn_hip_readset_results(galn_hip_readset *rs, galn_hip_out *out);
 /* Time of the last search's kernels in ms, measured with HIP events on the
  * search stream (first tier / all tiers). */
 int galn_hip_readset_kernel_ms(const galn_hip_readset *rs, double *first_tier_ms, double *total_ms);
+/* Per kernel launch of the last search: tier (0 = first), duration in ms (HIP events on the
+ * search stream) and the algorithmic index bytes of the reads it finished. Returns the number
+ * of launches (fills at most `cap`). */
+int galn_hip_readset_launches(const galn_hip_readset *rs, int cap, int32_t *tier, double *ms, uint64_t *algo_bytes);
+/* Counters of the wave-per-read engine in the last search: rounds, chains started, chains
+ * committed, steps committed, longest chain per round (summed), clock ticks in speculation
+ * and in commits (summed over waves), 0. */
+int galn_hip_readset_engine_stats(galn_hip_readset *rs, uint64_t out[8]);
+
+/* Random-access ceiling of `device`: pointer chase over a `ws_bytes` buffer of
+ * `line_bytes` (64 or 128) lines in one pseudo-random cycle, waves_per_cu * CUs
+ * independent chains of `iters` dependent whole-line loads each. Returns the
+ * saturated line gathers per second. No counterpart in the reference (measurement). */
+int galn_hip_gather_ceiling(int device, uint64_t ws_bytes, int line_bytes, int waves_per_cu, int iters,
+                            double *lines_per_s, double *ms);
 
 #ifdef __cplusplus
 }
