@@ -954,6 +954,7 @@ struct HShared {
     int has_pend, claim, status, result, K, b, J, jcap, jcut, tcut, n_piece, n_own, aleft;
     uint32_t abase;
     int nN;
+    long long ne_delta;
     uint64_t hk, hl;
     Ent pend, he;
     int he_score;
@@ -1365,19 +1366,17 @@ __global__ void __launch_bounds__(64) k_heavy(HArgs a)
                 }
                 F.jcut = jcut;
                 F.tcut = jcut >= 0 ? lcut[jcut] : 0;
-                // pops, trace and algorithmic bytes of the committed steps, in order
-                const int jend = jcut >= 0 ? jcut : last;
-                long nb = F.n_entries;
-                uint64_t tr = F.trace;
-                for (int j = 0; j <= jend; ++j) {
-                    const int cj = (j == jcut) ? F.tcut : lcommit[j];
-                    nb = lnstart[j];
-                    for (int t = 0; t < cj; ++t) {
-                        const uint32_t info = SINFO_ALL[j * H_STEPS + t];
-                        F.obytes += info >> 8;
-                        F.pops += 1;
-                        nb -= 1;
-                        if (TRACE) {
+                F.J = jcut >= 0 ? jcut : last;  // the committed chains are 0..J
+                F.ne_delta = 0;
+                if (TRACE) {
+                    // the trace hash runs over the committed pops in order
+                    uint64_t tr = F.trace;
+                    for (int j = 0; j <= F.J; ++j) {
+                        const int cj = (j == jcut) ? F.tcut : lcommit[j];
+                        long nb = lnstart[j];
+                        for (int t = 0; t < cj; ++t) {
+                            const uint32_t info = SINFO_ALL[j * H_STEPS + t];
+                            nb -= 1;
                             const Ent e = SLOG_ALL[j * H_STEPS + t];
                             const int i0 = e_i(e), mm = e_mm(e), go = e_go(e), ge = e_ge(e), st = e_st(e), ins = e_ins(e);
                             tr = tr_mix(tr, (uint64_t)((uint32_t)b << 21 | (uint32_t)i0));
@@ -1387,30 +1386,40 @@ __global__ void __launch_bounds__(64) k_heavy(HArgs a)
                             tr = tr_mix(tr, (uint64_t)ins | (uint64_t)(go + ge - ins) << 16);
                             tr = tr_mix(tr, (uint64_t)(uint32_t)e_ldp(e));
                             tr = tr_mix(tr, (uint64_t)(uint32_t)nb);
+                            nb += (info & 0xff) + 1;  // its pushes, and its exact child (popped next, if any)
                         }
-                        nb += (info & 0xff) + 1;  // its pushes, and its exact child (popped next, if any)
+                    }
+                    F.trace = tr;
+                }
+            }
+            __syncthreads();
+            // pops, algorithmic bytes and the change in entries of the committed steps, per lane
+            int ncp = 0;  // this lane's committed pushes
+            {
+                const int jend = F.J, jcut = F.jcut;
+                if (lane <= jend) {
+                    const int cj = (lane == jcut) ? F.tcut : lcommit[lane];
+                    uint64_t by = 0;
+                    for (int t = 0; t < cj; ++t) {
+                        by += SINFO[t] >> 8;
+                        ncp += SINFO[t] & 0xff;
+                    }
+                    if (cj > 0) {
+                        atomicAdd(&F.pops, (unsigned long long)cj);
+                        atomicAdd(&F.obytes, (unsigned long long)by);
+                        atomicAdd(reinterpret_cast<unsigned long long *>(&F.ne_delta), (unsigned long long)(long long)(ncp - 1));
                     }
                 }
-                F.trace = tr;
-                if (jcut >= 0) {
-                    F.status = 1;
-                    F.result = F.n_aln;
-                } else {
-                    F.J = last;  // from here on: the committed chains are 0..last
-                    int np_tot = 0;
-                    for (int j = 0; j <= last; ++j) np_tot += lcommit[j] == lsteps[j] ? lnp[j] : 0;
-                    (void)np_tot;
-                }
+            }
+            __syncthreads();
+            if (lane == 0 && F.jcut >= 0) {
+                F.status = 1;
+                F.result = F.n_aln;
             }
             __syncthreads();
             if (F.status) break;
             // ---- pushes: rank of every committed push within its bucket, then segments ----
             const int last = F.J;
-            int ncp = 0;  // this lane's committed pushes
-            if (lane <= last) {
-                const int c = lcommit[lane];
-                for (int t = 0; t < c; ++t) ncp += SINFO[t] & 0xff;
-            }
             for (int s = 0; s < NB; ++s) cnt[s * HW + lane] = 0;
             for (int u = 0; u < ncp; ++u) cnt[PSC[u] * HW + lane] += 1;
             __syncthreads();
@@ -1502,13 +1511,7 @@ __global__ void __launch_bounds__(64) k_heavy(HArgs a)
                     }
                 }
                 // entries: every committed pop -1, every committed push +1, a pending child +1
-                long ne = F.n_entries;
-                for (int j = 0; j <= last; ++j) {
-                    const int c = lcommit[j];
-                    int pj = 0;
-                    for (int t = 0; t < c; ++t) pj += SINFO_ALL[j * H_STEPS + t] & 0xff;
-                    ne += pj - 1;
-                }
+                long ne = F.n_entries + F.ne_delta;
                 const int lc = last >= 0 ? lcommit[last] : 0;
                 const bool partial = last >= 0 && lc < lsteps[last];
                 const bool trunc = last >= 0 && lc == lsteps[last] && lkind[last] == K_TRUNC;

@@ -1,0 +1,160 @@
+"""Parity of the HIP search with the reference, on the GPU (pytest -m gpu).
+
+Every golden run (the reference's own C search, committed in tests/golden/) is searched
+through the C ABI in each form -- plain, batched, batched with every read sent to the
+wave-per-read engine, batched without the exact-child register path, batched with one
+record slot per lane (forces the re-run path) -- and must give the golden .sai bytes, pop
+counts and trace hashes. The `galn` executable must write the golden .sai. Larger inputs
+are checked by size-independent properties: the forms agree bit for bit, and a sample
+agrees with the oracle."""
+from __future__ import annotations
+
+import subprocess
+
+import numpy as np
+import pytest
+
+import fixtures as fx
+import oracle as orc
+from conftest import golden_inputs, golden_runs, load_reads, run_opts
+from galn_amd import index as ix
+from galn_amd import native as nv
+
+pytestmark = pytest.mark.gpu
+
+
+def tune(**kw) -> nv.Tune:
+    t = nv.Tune()
+    t.sort_by_cost = 1
+    for k, v in kw.items():
+        setattr(t, k, v)
+    return t
+
+
+FORMS = {
+    "plain": (nv.FORM_PLAIN, 0, None),
+    "batched": (nv.FORM_BATCHED, 0, None),
+    "heavy_all": (nv.FORM_BATCHED, 0, dict(heavy_pops=1)),
+    "heavy_64": (nv.FORM_BATCHED, 0, dict(heavy_pops=64)),
+    "no_fuse": (nv.FORM_BATCHED, nv.GALN_HIP_NOFUSE, dict(heavy_pops=-1)),
+    "rec1": (nv.FORM_BATCHED, 0, dict(rec_per_lane=1, heavy_pops=-1)),
+    "unsorted_small_grid": (nv.FORM_BATCHED, 0, dict(sort_by_cost=0, waves_per_cu=1)),
+}
+
+_index_cache: dict = {}
+
+
+def readset_for(name, key, golden_dir):
+    w = golden_inputs(name, golden_dir)
+    opts, mate = run_opts(w["meta"], key)
+    r = load_reads(w["reads"][mate])
+    if name not in _index_cache:
+        meta, words = ix.read_bwt(w["bwt"])
+        _index_cache[name] = nv.Index(words, meta, 0)
+    o = nv.gap_opt_from_args(opts)
+    bopts, nst, md = nv.prepare_batches(o, r["qlen"], r["batch"])
+    rs = nv.ReadSet(_index_cache[name], r["query"], r["qoff"], r["qlen"], md, r["batch"], bopts, nst, o.seed_len)
+    return w, o, rs
+
+
+@pytest.mark.parametrize("form", list(FORMS))
+@pytest.mark.parametrize("name,key", golden_runs())
+def test_golden_sai_pops_trace(name, key, form, golden_dir):
+    print(f"case {name}/{key} form={form}", flush=True)
+    w, o, rs = readset_for(name, key, golden_dir)
+    f, flags, tk = FORMS[form]
+    rs.search(f, flags | nv.GALN_HIP_TRACE, tune(**tk) if tk is not None else None)
+    res = rs.results(want_pops=True, want_trace=True)
+    exp = w["expected"]
+    np.testing.assert_array_equal(res.n_aln, exp[key + "_n_aln"])
+    np.testing.assert_array_equal(res.pops, exp[key + "_pops"])
+    np.testing.assert_array_equal(res.trace, exp[key + "_trace"])
+    assert fx.md5_bytes(nv.sai_bytes(o, res.n_aln, res.rec_off, res.rec)) == w["meta"]["runs"][key]["sai_md5"]
+
+
+@pytest.mark.parametrize("form", ["plain", "batched"])
+@pytest.mark.parametrize("name,key", [("plain_small", "default_1"), ("adna_mixed", "adna_1"), ("deep_repeats", "deep_2"),
+                                      ("options", "max_entries_m300_1"), ("fastq_edge", "adna_1"),
+                                      ("serving_mix", "default_1")])
+def test_cli_writes_golden_sai(name, key, form, golden_dir, tmp_path):
+    print(f"case cli {name}/{key} form={form}", flush=True)
+    w = golden_inputs(name, golden_dir)
+    opts, mate = run_opts(w["meta"], key)
+    out = tmp_path / "out.sai"
+    r = subprocess.run([nv.GALN_BIN, "aln", "--form", form, *opts, "-f", str(out), w["prefix"], w["reads"][mate]],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    assert fx.md5_file(str(out)) == w["meta"]["runs"][key]["sai_md5"]
+
+
+def test_algorithmic_bytes_match_oracle(golden_dir):
+    """The kernel's per-read algorithmic byte count is the oracle's (same definition)."""
+    w, o, rs = readset_for("adna_mixed", "adna_1", golden_dir)
+    rs.search(nv.FORM_BATCHED)
+    res = rs.results(want_bytes=True)
+    r = load_reads(w["reads"][0])
+    meta, words = ix.read_bwt(w["bwt"])
+    oo = orc.parse_aln_opts(w["meta"]["runs"]["adna_1"]["opts"])
+    bopts, nst, maxd = orc.batch_options(oo, r["qlen"], r["batch"])
+    ref = orc.search(words, meta, r["query"], r["qoff"], r["qlen"], maxd, r["batch"], bopts, nst, oo["seed_len"])
+    np.testing.assert_array_equal(res.occ_bytes, ref.occ_bytes)
+    launches = rs.launches()
+    assert sum(l["algo_bytes"] for l in launches) == int(ref.occ_bytes.sum())
+
+
+def test_repeated_searches_identical(golden_dir):
+    w, o, rs = readset_for("human_like", "adna_1", golden_dir)
+    outs = []
+    for form in (nv.FORM_BATCHED, nv.FORM_PLAIN, nv.FORM_BATCHED):
+        rs.search(form)
+        res = rs.results()
+        outs.append(nv.sai_bytes(o, res.n_aln, res.rec_off, res.rec))
+    assert outs[0] == outs[1] == outs[2]
+
+
+def test_reads_outside_limits_are_reported():
+    """A read longer than the kernel's limit gets the unsupported code, not a wrong answer."""
+    from galn_amd import synth
+    ref = synth.plain_reference(5, 20000)
+    meta = ix.build_index(ref.tobytes(), "/tmp/galn_lim")
+    meta, words = ix.read_bwt("/tmp/galn_lim.bwt")
+    reads = np.zeros((3, 1100), np.uint8)
+    lens = np.array([50, 1100, 80])
+    for i, L in enumerate(lens):
+        reads[i, :L] = ref[100 * i:100 * i + L]
+    q, qo = synth.encode_queries(reads, lens)
+    o = nv.gap_opt_from_args([])
+    batch = np.zeros(3, np.int32)
+    bopts, nst, md = nv.prepare_batches(o, lens.astype(np.int32), batch)
+    with nv.Index(words, meta, 0) as idx:
+        rs = nv.ReadSet(idx, q, qo, lens.astype(np.int32), md, batch, bopts, nst, o.seed_len)
+        for form in (nv.FORM_PLAIN, nv.FORM_BATCHED):
+            rs.search(form)
+            res = rs.results()
+            assert res.n_aln[1] == -5
+            assert res.n_aln[0] >= 1 and res.n_aln[2] >= 1
+
+
+def test_larger_reference_forms_agree_and_sample_matches_oracle(tmp_path):
+    """100 Mbp aDNA-like reference (index built on the GPU), 100k reads: plain and batched
+    agree bit for bit, and 1500 reads agree with the oracle."""
+    from galn_amd import workloads
+    print("case large: building 100 Mbp workload", flush=True)
+    w = workloads.adna_mixed(str(tmp_path), ref_bp=100_000_000, n_reads=100_000, device="cuda", workers=8)
+    with nv.Index(w.words, w.meta, 0) as idx:
+        rs = w.readset(idx)
+        rs.search(nv.FORM_BATCHED)
+        rb = rs.results()
+        rs.search(nv.FORM_PLAIN)
+        rp = rs.results()
+    np.testing.assert_array_equal(rb.n_aln, rp.n_aln)
+    np.testing.assert_array_equal(rb.rec, rp.rec)
+    o = orc.parse_aln_opts(w.opts)
+    n = 1500
+    bid = orc.batch_ids(n)
+    bopts, nst, maxd = orc.batch_options(o, w.lens[:n].astype(np.int32), bid)
+    ref = orc.search(w.words, w.meta, w.query, w.qoff[:n], w.lens[:n].astype(np.int32), maxd, bid, bopts, nst,
+                     o["seed_len"], 16)
+    np.testing.assert_array_equal(rb.n_aln[:n], ref.n_aln)
+    for i in range(n):
+        np.testing.assert_array_equal(rb.records(i), ref.records(i))
