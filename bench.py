@@ -100,6 +100,95 @@ def cpu_baseline(w, res, sample: int, threads: int, workdir: str) -> dict:
             "records_equal_gpu": agree}
 
 
+def serving_sweep(w, index, nv, loads, seconds: float, max_req: int, seed: int = 7) -> dict:
+    """Serving loop under Poisson arrivals. Requests are slices of the workload's reads of
+    log-uniform size (1000 .. max_req reads), each searched with BWA's default options or the
+    aDNA options (50/50), batch options computed per request as `galn aln` would. The
+    server's peak rate is measured first (requests submitted back to back); each offered
+    load then submits requests at that fraction of the peak for `seconds`, and reports the
+    p50/p99 latency (submit to the request's records ready) and the reads/s served."""
+    import threading
+
+    rng = np.random.default_rng(seed)
+    opt_sets = {"default": nv.gap_opt_from_args([]), "adna": nv.gap_opt_from_args(list(w.opts))}
+    lens32 = w.lens.astype(np.int32)
+
+    def make_request():
+        size = int(np.exp(rng.uniform(np.log(1000), np.log(max_req))))
+        size = min(size, w.n)
+        lo = int(rng.integers(0, w.n - size + 1))
+        hi = lo + size
+        o = opt_sets["default" if rng.random() < 0.5 else "adna"]
+        ql = lens32[lo:hi]
+        bid = (np.arange(size) // 0x40000).astype(np.int32)
+        bopts, nst, md = nv.prepare_batches(o, ql, bid)
+        q0 = int(w.qoff[lo])
+        q1 = int(w.qoff[hi - 1]) + int(ql[-1])
+        return (w.query[q0:q1], w.qoff[lo:hi] - q0, ql, md, bid, bopts, nst)
+
+    def run(srv, reqs, rate):
+        """Submit reqs (at `rate` requests/s, or back to back if None); latencies in ms."""
+        ids, lats, t_sub = [], [], []
+        done = threading.Event()
+
+        def collect():
+            k = 0
+            while k < len(reqs):
+                while k >= len(ids):
+                    time.sleep(0.0005)
+                srv.wait(ids[k], -1)
+                _, lat = srv.result(ids[k])
+                lats.append(lat)
+                k += 1
+            done.set()
+
+        th = threading.Thread(target=collect, daemon=True)
+        th.start()
+        t0 = time.perf_counter()
+        nxt = t0
+        for r in reqs:
+            if rate:
+                nxt += float(rng.exponential(1.0 / rate))
+                d = nxt - time.perf_counter()
+                if d > 0:
+                    time.sleep(d)
+            while True:
+                rid = srv.submit(*r)
+                if rid is not None:
+                    break
+                time.sleep(0.001)  # admission limit: retry shortly
+            ids.append(rid)
+            t_sub.append(time.perf_counter())
+        done.wait()
+        return lats, time.perf_counter() - t0
+
+    out = {"request_reads": f"log-uniform 1000..{max_req}", "options": "50/50 default / aln -l 1024 -n 0.01 -o 2",
+           "loads": []}
+    with nv.Server(index) as srv:
+        warm = [make_request() for _ in range(2)]
+        run(srv, warm, None)
+        peak_reqs = [make_request() for _ in range(12)]
+        lats, el = run(srv, peak_reqs, None)
+        n_reads = sum(int(r[2].size) for r in peak_reqs)
+        peak = n_reads / el
+        mean_size = n_reads / len(peak_reqs)
+        out["peak_reads_per_s"] = peak
+        log(f"serving: peak {peak:.3e} reads/s over {len(peak_reqs)} requests")
+        for load in loads:
+            rate = load * peak / mean_size
+            n_req = max(4, int(rate * seconds))
+            reqs = [make_request() for _ in range(n_req)]
+            lats, el = run(srv, reqs, rate)
+            reads = sum(int(r[2].size) for r in reqs)
+            rec = {"offered": load, "requests": n_req, "reads_per_s": reads / el,
+                   "p50_ms": float(np.percentile(lats, 50)), "p99_ms": float(np.percentile(lats, 99))}
+            out["loads"].append(rec)
+            log(f"serving: load {load:.2f}: {n_req} requests, p50 {rec['p50_ms']:.1f} ms, p99 {rec['p99_ms']:.1f} ms, "
+                f"{rec['reads_per_s']:.3e} reads/s")
+        out["server_stats"] = srv.stats()
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -111,6 +200,8 @@ def main() -> int:
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--serve-seconds", type=float, default=8.0, help="arrival time per offered load (0: no serving)")
+    ap.add_argument("--serve-max-request", type=int, default=1_000_000)
     ap.add_argument("--workdir", default="")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -218,6 +309,11 @@ def main() -> int:
         "ceiling": {"kind": "pointer chase, 64-byte random lines", "ws_bytes": ws, "GB/s": ceiling_gbs,
                     "frac": achieved / ceiling_gbs if ceiling_gbs else None},
     }
+    if args.serve_seconds > 0:
+        out["serving"] = serving_sweep(w, index, nv, (0.25, 0.5, 0.75, 0.9), args.serve_seconds,
+                                       min(args.serve_max_request, w.n))
+    rs.close()
+    index.close()
     if not args.no_cpu:
         sample = args.cpu_sample
         if sample <= 0:

@@ -33,6 +33,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cstdarg>
 #include <cstdint>
 #include <cstdio>
@@ -91,7 +92,6 @@ struct KArgs {
     const int32_t *batch;
     const galn_search_opt_t *bopts;
     const int32_t *nstacks;
-    int32_t seed_len;
     // work: list positions [0, n_list); read id = list ? list[pos] : list_base + pos
     const int32_t *list;
     int32_t list_base;
@@ -109,7 +109,7 @@ struct KArgs {
     Ent *pool_chk;           // n_gran * GRAN * CH entries
     uint32_t *pool_link;     // per chunk: previous chunk of its bucket, or next free chunk of its lane
     uint32_t *gnext;         // per granule: next granule in the free stack
-    unsigned long long *ghead;  // free stack head: tag << 32 | granule (0xffffffff = empty)
+    unsigned long long *ghead;  // [NSHARD] free stack heads: tag << 32 | granule (0xffffffff = empty)
     uint32_t *gbump;         // granules never handed out start here
     uint32_t n_gran;
     // results (indexed by read id)
@@ -227,36 +227,52 @@ __device__ __forceinline__ uint64_t wpack(uint64_t w, int bid) { return w | ((ui
 
 enum : int { PH_CW = 0, PH_POPPED = 1, PH_EXACT = 2 };
 
-// Free granules: a lock-free stack whose head carries a tag bumped on every change
-// (no ABA), links read and written at device scope; then never-used granules.
-__device__ __forceinline__ uint32_t gran_pop(unsigned long long *ghead, uint32_t *gnext, uint32_t *gbump, uint32_t n_gran)
+// Free granules: NSHARD lock-free stacks, each with a head that carries a tag bumped on every
+// change (no ABA), links read and written at device scope; then never-used granules. A
+// workgroup uses the shard of its index, so contention on one head stays low.
+constexpr int NSHARD = 64;
+
+__device__ __forceinline__ uint32_t shard_pop(unsigned long long *head, uint32_t *gnext)
 {
     for (;;) {
-        unsigned long long old = __hip_atomic_load(ghead, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+        unsigned long long old = __hip_atomic_load(head, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
         const uint32_t g = (uint32_t)old;
-        if (g == NIL) break;
+        if (g == NIL) return NIL;
         const uint32_t nxt = __hip_atomic_load(gnext + g, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         const unsigned long long nw = ((old >> 32) + 1) << 32 | nxt;
-        if (__hip_atomic_compare_exchange_strong(ghead, &old, nw, __ATOMIC_ACQ_REL, __ATOMIC_RELAXED,
+        if (__hip_atomic_compare_exchange_strong(head, &old, nw, __ATOMIC_ACQ_REL, __ATOMIC_RELAXED,
                                                  __HIP_MEMORY_SCOPE_AGENT))
             return g;
     }
-    const uint32_t b = atomicAdd(gbump, 1u);
-    return b < n_gran ? b : NIL;
 }
 
-// push the chain own[from..to) (linked through gnext) in one exchange
+__device__ __forceinline__ uint32_t gran_pop(unsigned long long *ghead, uint32_t *gnext, uint32_t *gbump, uint32_t n_gran)
+{
+    const int sh = (int)(blockIdx.x & (NSHARD - 1));
+    uint32_t g = shard_pop(ghead + sh, gnext);
+    if (g != NIL) return g;
+    if (__hip_atomic_load(gbump, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < n_gran) {
+        const uint32_t b = atomicAdd(gbump, 1u);
+        if (b < n_gran) return b;
+    }
+    // the never-used granules are gone: look in the other shards
+    for (int t = 1; t < NSHARD && g == NIL; ++t) g = shard_pop(ghead + ((sh + t) & (NSHARD - 1)), gnext);
+    return g;
+}
+
+// push the chain own[from..to) (linked through gnext) onto this workgroup's shard in one exchange
 __device__ __forceinline__ void gran_push(unsigned long long *ghead, uint32_t *gnext, const uint32_t *own, int from,
                                           int to)
 {
+    unsigned long long *const head = ghead + (blockIdx.x & (NSHARD - 1));
     for (int j = from; j + 1 < to; ++j)
         __hip_atomic_store(gnext + own[j], own[j + 1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     const uint32_t first = own[from], last = own[to - 1];
-    unsigned long long old = __hip_atomic_load(ghead, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    unsigned long long old = __hip_atomic_load(head, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     for (;;) {
         __hip_atomic_store(gnext + last, (uint32_t)old, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         const unsigned long long nw = ((old >> 32) + 1) << 32 | first;
-        if (__hip_atomic_compare_exchange_strong(ghead, &old, nw, __ATOMIC_RELEASE, __ATOMIC_RELAXED,
+        if (__hip_atomic_compare_exchange_strong(head, &old, nw, __ATOMIC_RELEASE, __ATOMIC_RELAXED,
                                                  __HIP_MEMORY_SCOPE_AGENT))
             return;
     }
@@ -288,7 +304,7 @@ __global__ void __launch_bounds__(128) k_search(KArgs a)
     int s_mm = 0, s_gapo = 0, s_gape = 0, mode = 0, ies = 0, mdo = 0, max_entries = 0, max_gapo = 0, max_gape = 0,
         msd = 0, max_top2 = 0, n_stacks = 0, md_read = 0;
     bool seeding = false;
-    int seed_len = a.seed_len;
+    int seed_len = 0;
     // chain (widths / exact tail)
     int phase = PH_CW, cj = 0, plen = 0, cbid = 0, cbase = 0;
     const uint8_t *pstr = nullptr;
@@ -678,6 +694,7 @@ __global__ void __launch_bounds__(128) k_search(KArgs a)
         max_top2 = o.max_top2;
         n_stacks = a.nstacks[b];
         md_read = a.maxdiff[rid];
+        seed_len = o.seed_len;
         seeding = seed_len < len;
         pops = 0;
         trace = TR_INIT;
@@ -903,6 +920,7 @@ constexpr int HW = 64;         // lanes, one wave per workgroup
 constexpr int H_STEPS = 32;    // chain steps per lane per round
 constexpr int H_PUSH = 128;    // logged pushes per lane per round
 constexpr int H_CAP = 4096;    // pushes committed per round
+constexpr int H_KEEP = 64;     // granules a wave keeps between reads
 constexpr int H_PIECES = MAXB + 16;
 constexpr uint32_t GENT = (uint32_t)GRAN * CH;  // entries per granule
 
@@ -918,14 +936,13 @@ struct HArgs {
     const int32_t *batch;
     const galn_search_opt_t *bopts;
     const int32_t *nstacks;
-    int32_t seed_len;
     const int32_t *list;
     int32_t n_list;
     int32_t *qnext;
     int32_t n_slots;
     uint8_t *scratch;
     uint64_t slot_bytes;
-    uint32_t rec_off, own_off, plog_off, pscore_off, slog_off, sinfo_off;
+    uint32_t rec_off, own_off, plog_off, slog_off, sinfo_off;
     int32_t own_cap, rec_cap;
     Ent *pool;
     uint32_t *gnext;
@@ -946,15 +963,16 @@ struct HArgs {
 };
 
 struct HShared {
-    int rid, len, n_stacks, md_read, seeding;
+    int rid, len, n_stacks, md_read, seeding, seed_len;
     int s_mm, s_gapo, s_gape, mode, ies, mdo, max_entries, max_gapo, max_gape, msd, max_top2;
     int best_score, best_diff, max_diff, best_cnt, n_aln, n_entries;
     unsigned long long pops, trace, obytes;
     unsigned long long bm[4];
-    int has_pend, claim, status, result, K, b, J, jcap, jcut, tcut, n_piece, n_own, aleft;
+    int has_pend, claim, status, result, K, b, J, jcap, jcut, tcut, n_piece, n_own, n_used, aleft;
     uint32_t abase;
     int nN;
-    long long ne_delta;
+    int l_partial, l_full;  // the last committed chain: cut inside, or its stop kind when whole (-1)
+    unsigned long long st[16];
     uint64_t hk, hl;
     Ent pend, he;
     int he_score;
@@ -964,6 +982,36 @@ __device__ __forceinline__ int hbm_low(const unsigned long long *bm)
 {
     return bm[0] ? __builtin_ctzll(bm[0]) : bm[1] ? 64 + __builtin_ctzll(bm[1]) : bm[2] ? 128 + __builtin_ctzll(bm[2])
                  : bm[3] ? 192 + __builtin_ctzll(bm[3]) : -1;
+}
+
+// wave-wide (64-lane) scans and reductions; every lane of the wave must call them
+__device__ __forceinline__ int wave_excl_scan(int v)
+{
+    const int lane = threadIdx.x & (HW - 1);
+    int x = v;
+    for (int d = 1; d < HW; d <<= 1) {
+        const int t = __shfl_up(x, d, HW);
+        if (lane >= d) x += t;
+    }
+    return x - v;
+}
+
+__device__ __forceinline__ int wave_sum_i32(int v)
+{
+    for (int d = HW / 2; d > 0; d >>= 1) v += __shfl_xor(v, d, HW);
+    return v;
+}
+
+__device__ __forceinline__ unsigned long long wave_sum_u64(unsigned long long v)
+{
+    for (int d = HW / 2; d > 0; d >>= 1) v += __shfl_xor(v, d, HW);
+    return v;
+}
+
+__device__ __forceinline__ int wave_max_i32(int v)
+{
+    for (int d = HW / 2; d > 0; d >>= 1) v = max(v, __shfl_xor(v, d, HW));
+    return v;
 }
 
 // bwt_cal_width into LDS by one lane (src/bwtgap_hd.h:216-242); returns the algorithmic bytes
@@ -999,17 +1047,15 @@ __global__ void __launch_bounds__(64) k_heavy(HArgs a)
     const int NB = a.buckets;
     uint32_t *seg_base = reinterpret_cast<uint32_t *>(lp);
     uint32_t *seg_cnt = seg_base + NB;
-    uint32_t *Tb = seg_cnt + NB;
-    int32_t *first_piece = reinterpret_cast<int32_t *>(Tb + NB);
-    uint16_t *cnt = reinterpret_cast<uint16_t *>(first_piece + NB);  // cnt[s * 64 + lane]
-    lp = reinterpret_cast<uint8_t *>(cnt + NB * HW);
+    int32_t *first_piece = reinterpret_cast<int32_t *>(seg_cnt + NB);
+    lp = reinterpret_cast<uint8_t *>(first_piece + NB);
     lp = hlds + (((size_t)(lp - hlds) + 15) & ~(size_t)15);
     uint4 *pieces = reinterpret_cast<uint4 *>(lp);  // {bucket, rank0, len, base}
     uint64_t *W = reinterpret_cast<uint64_t *>(pieces + H_PIECES);
     uint64_t *SW = W + a.wlen;
-    int32_t *lsteps = reinterpret_cast<int32_t *>(SW + a.swlen);
-    int32_t *lnp = lsteps + HW, *lkind = lnp + HW, *lcommit = lkind + HW, *lpexcl = lcommit + HW, *lnstart = lpexcl + HW;
-    int32_t *lcut = lnstart + HW;
+    int32_t *lcommit = reinterpret_cast<int32_t *>(SW + a.swlen);  // per chain, for the trace hash
+    int32_t *lnstart = lcommit + HW;
+    uint8_t *const PSC = reinterpret_cast<uint8_t *>(lnstart + HW) + threadIdx.x * H_PUSH;  // bucket of each logged push
 
     const int lane = threadIdx.x;
     if ((int)blockIdx.x >= a.n_slots) return;
@@ -1017,7 +1063,6 @@ __global__ void __launch_bounds__(64) k_heavy(HArgs a)
     galn_aln_rec_t *const REC = reinterpret_cast<galn_aln_rec_t *>(base + a.rec_off);
     uint32_t *const OWN = reinterpret_cast<uint32_t *>(base + a.own_off);
     Ent *const PLOG = reinterpret_cast<Ent *>(base + a.plog_off) + lane * H_PUSH;
-    uint8_t *const PSC = base + a.pscore_off + lane * H_PUSH;
     Ent *const SLOG_ALL = reinterpret_cast<Ent *>(base + a.slog_off);
     uint32_t *const SINFO_ALL = reinterpret_cast<uint32_t *>(base + a.sinfo_off);
     Ent *const SLOG = SLOG_ALL + lane * H_STEPS;
@@ -1026,6 +1071,10 @@ __global__ void __launch_bounds__(64) k_heavy(HArgs a)
     const DMeta &M = a.meta;
     const uint64_t seq_len = M.seq_len;
 
+    if (lane == 0) {
+        for (int j = 0; j < 16; ++j) F.st[j] = 0;
+        F.n_own = 0;  // granules this wave holds; a read reuses them before taking more
+    }
     for (;;) {
         if (lane == 0) F.claim = atomicAdd(a.qnext, 1);
         __syncthreads();
@@ -1053,10 +1102,11 @@ __global__ void __launch_bounds__(64) k_heavy(HArgs a)
             F.max_top2 = o.max_top2;
             F.n_stacks = a.nstacks[b0];
             F.md_read = a.maxdiff[rid];
-            F.seeding = a.seed_len < len;
+            F.seed_len = o.seed_len;
+            F.seeding = o.seed_len < len;
             F.nN = 0;
             F.obytes = 0;
-            F.n_own = 0;
+            F.n_used = 0;
             F.aleft = 0;
             F.abase = 0;
             F.status = 0;
@@ -1067,7 +1117,7 @@ __global__ void __launch_bounds__(64) k_heavy(HArgs a)
         if (!unsup) {
             uint64_t wb = 0;
             if (lane == 0) wb = cal_width_one(a.bwt, M, len, q, W);
-            if (lane == 1 && F.seeding) wb = cal_width_one(a.bwt, M, a.seed_len, q + (len - a.seed_len), SW);
+            if (lane == 1 && F.seeding) wb = cal_width_one(a.bwt, M, F.seed_len, q + (len - F.seed_len), SW);
             int nN = 0;
             for (int j = lane; j < len; j += HW) nN += q[j] > 3;
             if (nN) atomicAdd(&F.nN, nN);
@@ -1146,10 +1196,10 @@ __global__ void __launch_bounds__(64) k_heavy(HArgs a)
             if (F.status) break;
             // ---- speculation: lane j runs the chain of the j-th entry from the top of bucket b ----
             const int b = F.b, K = F.K, hp = F.has_pend;
-            const long long t_spec = wall_clock64();
+            const long long t_spec = a.stats ? wall_clock64() : 0;
             const int s_mm = F.s_mm, s_gapo = F.s_gapo, s_gape = F.s_gape, mode = F.mode, max_diff = F.max_diff;
             const int n_stacks = F.n_stacks, max_gapo = F.max_gapo, max_gape = F.max_gape, ies = F.ies, mdo = F.mdo;
-            const int msd = F.msd, seed_len = a.seed_len;
+            const int msd = F.msd, seed_len = F.seed_len;
             const bool seeding = F.seeding != 0;
             int steps = 0, np = 0, kind = K_END;
             uint64_t hk = 0, hl = 0;
@@ -1297,84 +1347,82 @@ __global__ void __launch_bounds__(64) k_heavy(HArgs a)
                     e = child;
                 }
             }
-            lsteps[lane] = steps;
-            lnp[lane] = np;
-            lkind[lane] = lane < K ? kind : K_END;
-            __syncthreads();
-            const long long t_commit = wall_clock64();
-            // ---- commit plan (lane 0): first stopping chain, push cap, entry counts ----
-            if (lane == 0) {
-                int J = K - 1;
-                for (int j = 0; j < K; ++j)
-                    if (lkind[j] != K_END) {
-                        J = j;
+            // ---- commit plan, by wave-wide scans over the chains ----
+            if (lane >= K) {
+                steps = 0;
+                np = 0;
+                kind = K_END;
+            }
+            const long long t_commit = a.stats ? wall_clock64() : 0;
+            // J: the first chain that stops (a hit, a truncation, a score error); the chains after
+            // it are re-run in the next round
+            const uint64_t stopm = __ballot(kind != K_END);
+            const int J = stopm ? __builtin_ctzll(stopm) : K - 1;
+            const bool inJ = lane <= J;
+            const int pex = wave_excl_scan(inJ ? np : 0);  // pushes of the chains before this one
+            // entries before this chain's first pop: each chain pops its head, pushes its logged
+            // children and, when truncated, leaves its next exact child pending
+            const int nstart = F.n_entries + wave_excl_scan(inJ ? np - 1 + (kind == K_TRUNC ? 1 : 0) : 0);
+            // the push cap falls in chain jcap, which commits whole steps up to it
+            const uint64_t capm = __ballot(inJ && pex + np > H_CAP);
+            const int jcap = capm ? __builtin_ctzll(capm) : J + 1;
+            const bool cand = inJ && lane <= jcap;
+            int c = 0, cut = -1;
+            if (cand) {
+                c = steps;
+                if (lane == jcap) {
+                    int acc = pex;
+                    c = 0;
+                    while (c < steps && acc + (int)(SINFO[c] & 0xff) <= H_CAP) acc += SINFO[c++] & 0xff;
+                }
+                // the max_entries check before each pop (src/bwtgap_hd.h:360)
+                long nb = nstart;
+                for (int t = 0; t < c; ++t) {
+                    if (nb > F.max_entries) {
+                        cut = t;
                         break;
                     }
-                int pex = 0, jcap = J + 1;
-                long nst = F.n_entries;
-                for (int j = 0; j <= J; ++j) {
-                    lpexcl[j] = pex;
-                    lnstart[j] = (int)nst;
-                    if (pex + lnp[j] > H_CAP && jcap > J) jcap = j;
-                    pex += lnp[j];
-                    nst += lnp[j] - 1 + (lkind[j] == K_TRUNC ? 1 : 0);
+                    nb += SINFO[t] & 0xff;
                 }
-                F.J = J;
-                F.jcap = jcap;
-            }
-            __syncthreads();
-            // ---- each committed lane: steps it commits, and the max_entries check before each pop ----
-            {
-                const int J = F.J, jcap = F.jcap;
-                int c = 0, cut = -1;
-                if (lane <= J && lane <= jcap) {
-                    c = steps;
-                    if (lane == jcap) {
-                        // the push cap falls inside this chain: commit whole steps up to it
-                        int acc = lpexcl[lane];
-                        c = 0;
-                        while (c < steps && acc + (int)(SINFO[c] & 0xff) <= H_CAP) acc += SINFO[c++] & 0xff;
-                    }
-                    long nb = lnstart[lane];
-                    for (int t = 0; t < c; ++t) {
-                        if (nb > F.max_entries) {
-                            cut = t;
-                            break;
-                        }
-                        nb += SINFO[t] & 0xff;
-                    }
-                }
-                lcommit[lane] = c;
-                lcut[lane] = cut;
-                if (lane <= J && lane <= jcap && c > 0 && (kind == K_HIT || kind == K_ESCORE) && c == steps) {
+                if (c > 0 && c == steps && (kind == K_HIT || kind == K_ESCORE)) {
                     F.hk = hk;
                     F.hl = hl;
                     F.he = SLOG[steps - 1];
                 }
-                if (lane <= J && lane <= jcap && c > 0 && c == steps && kind == K_TRUNC) F.pend = nxt;
+                if (c > 0 && c == steps && kind == K_TRUNC) F.pend = nxt;
+            }
+            // the search stops inside chain jcut (max_entries); otherwise chains 0..last commit
+            const uint64_t cutm = __ballot(cand && cut >= 0);
+            const int jcut = cutm ? __builtin_ctzll(cutm) : -1;
+            const uint64_t posm = __ballot(cand && c > 0);
+            const int last = jcut >= 0 ? jcut : (posm ? 63 - __builtin_clzll(posm) : -1);
+            const int cj = lane <= last ? (lane == jcut ? cut : c) : 0;  // committed steps of this chain
+            int ncp = 0;  // committed pushes of this chain
+            uint64_t by = 0;
+            for (int t = 0; t < cj; ++t) {
+                by += SINFO[t] >> 8;
+                ncp += SINFO[t] & 0xff;
+            }
+            const unsigned long long sum_pops = wave_sum_u64(cj), sum_by = wave_sum_u64(by);
+            const int sum_ne = wave_sum_i32(cj > 0 ? ncp - 1 : 0);
+            if (TRACE) {
+                lcommit[lane] = cj;
+                lnstart[lane] = nstart;
+            }
+            if (lane == last) {
+                F.l_partial = c < steps;
+                F.l_full = c == steps ? kind : -1;
+                if (c < steps) F.pend = SLOG[c];
             }
             __syncthreads();
+            const long long t_c1 = a.stats ? wall_clock64() : 0;
             if (lane == 0) {
-                // the last committed chain, and where the search stops inside the round, if it does
-                int last = -1, jcut = -1;
-                for (int j = 0; j <= F.J && j <= F.jcap; ++j) {
-                    if (lcut[j] >= 0) {
-                        jcut = j;
-                        break;
-                    }
-                    if (lcommit[j] > 0) last = j;
-                }
-                F.jcut = jcut;
-                F.tcut = jcut >= 0 ? lcut[jcut] : 0;
-                F.J = jcut >= 0 ? jcut : last;  // the committed chains are 0..J
-                F.ne_delta = 0;
                 if (TRACE) {
                     // the trace hash runs over the committed pops in order
                     uint64_t tr = F.trace;
-                    for (int j = 0; j <= F.J; ++j) {
-                        const int cj = (j == jcut) ? F.tcut : lcommit[j];
+                    for (int j = 0; j <= last; ++j) {
                         long nb = lnstart[j];
-                        for (int t = 0; t < cj; ++t) {
+                        for (int t = 0; t < lcommit[j]; ++t) {
                             const uint32_t info = SINFO_ALL[j * H_STEPS + t];
                             nb -= 1;
                             const Ent e = SLOG_ALL[j * H_STEPS + t];
@@ -1391,69 +1439,63 @@ __global__ void __launch_bounds__(64) k_heavy(HArgs a)
                     }
                     F.trace = tr;
                 }
-            }
-            __syncthreads();
-            // pops, algorithmic bytes and the change in entries of the committed steps, per lane
-            int ncp = 0;  // this lane's committed pushes
-            {
-                const int jend = F.J, jcut = F.jcut;
-                if (lane <= jend) {
-                    const int cj = (lane == jcut) ? F.tcut : lcommit[lane];
-                    uint64_t by = 0;
-                    for (int t = 0; t < cj; ++t) {
-                        by += SINFO[t] >> 8;
-                        ncp += SINFO[t] & 0xff;
-                    }
-                    if (cj > 0) {
-                        atomicAdd(&F.pops, (unsigned long long)cj);
-                        atomicAdd(&F.obytes, (unsigned long long)by);
-                        atomicAdd(reinterpret_cast<unsigned long long *>(&F.ne_delta), (unsigned long long)(long long)(ncp - 1));
-                    }
+                F.pops += sum_pops;
+                F.obytes += sum_by;
+                if (jcut >= 0) {
+                    F.status = 1;
+                    F.result = F.n_aln;
                 }
-            }
-            __syncthreads();
-            if (lane == 0 && F.jcut >= 0) {
-                F.status = 1;
-                F.result = F.n_aln;
             }
             __syncthreads();
             if (F.status) break;
-            // ---- pushes: rank of every committed push within its bucket, then segments ----
-            const int last = F.J;
-            for (int s = 0; s < NB; ++s) cnt[s * HW + lane] = 0;
-            for (int u = 0; u < ncp; ++u) cnt[PSC[u] * HW + lane] += 1;
-            __syncthreads();
-            for (int s = lane; s < NB; s += HW) {
-                uint32_t run = 0;
-                for (int j = 0; j < HW; ++j) {
-                    const uint32_t v = cnt[s * HW + j];
-                    cnt[s * HW + j] = (uint16_t)run;
-                    run += v;
-                }
-                Tb[s] = run;
+            // ---- pushes: a committed push goes to bucket b + s_mm, b + s_gapo or b + s_gape
+            // (a mismatch, a gap open, a gap extension); its rank in the bucket is its place in
+            // the sequential push order, chain by chain ----
+            const int tg0 = b + s_mm, tg1 = b + s_gapo, tg2 = b + s_gape;
+            int n0 = 0, n1 = 0, n2 = 0, bad = 0;
+            for (int u = 0; u < ncp; ++u) {
+                const int sc = PSC[u];
+                if (sc == tg0) ++n0;
+                else if (sc == tg1) ++n1;
+                else if (sc == tg2) ++n2;
+                else bad = 1;
             }
-            __syncthreads();
+            const int r0 = wave_excl_scan(n0), r1 = wave_excl_scan(n1), r2 = wave_excl_scan(n2);
+            const int T0 = __shfl(r0 + n0, HW - 1), T1 = __shfl(r1 + n1, HW - 1), T2 = __shfl(r2 + n2, HW - 1);
+            const bool badw = __ballot(bad) != 0;
             if (lane == 0) {
+                // pieces of the wave's current granule for each bucket that receives pushes; a
+                // bucket's new piece links to its previous top segment
                 int np_ = 0;
                 bool oom = false;
-                for (int s = 0; s < NB && !oom; ++s) {
-                    uint32_t rem = Tb[s];
+                const int tgs[3] = {tg0, tg1, tg2};
+                const int tot[3] = {T0, T1, T2};
+                for (int x = 0; x < 3 && !oom && !badw; ++x) {
+                    const int sb = tgs[x];
+                    uint32_t rem = (uint32_t)tot[x];
                     if (!rem) continue;
-                    const bool bit = (F.bm[s >> 6] >> (s & 63)) & 1ull;
-                    uint32_t prev_base = bit ? seg_base[s] : NIL, prev_size = bit ? seg_cnt[s] : 0u, r0 = 0;
-                    first_piece[s] = np_;
+                    const bool bit = (F.bm[sb >> 6] >> (sb & 63)) & 1ull;
+                    uint32_t prev_base = bit ? seg_base[sb] : NIL, prev_size = bit ? seg_cnt[sb] : 0u, rk = 0;
+                    first_piece[sb] = np_;
                     while (rem > 0) {
                         if (F.aleft < 2) {
-                            if (F.n_own >= a.own_cap) {
-                                oom = true;
-                                break;
+                            uint32_t g;
+                            if (F.n_used < F.n_own) {
+                                g = OWN[F.n_used];
+                            } else {
+                                if (F.n_own >= a.own_cap) {
+                                    oom = true;
+                                    break;
+                                }
+                                g = gran_pop(a.ghead, a.gnext, a.gbump, a.n_gran);
+                                if (g == NIL) {
+                                    oom = true;
+                                    break;
+                                }
+                                if (a.stats) F.st[11] += 1;
+                                OWN[F.n_own++] = g;
                             }
-                            const uint32_t g = gran_pop(a.ghead, a.gnext, a.gbump, a.n_gran);
-                            if (g == NIL) {
-                                oom = true;
-                                break;
-                            }
-                            OWN[F.n_own++] = g;
+                            ++F.n_used;
                             F.abase = g * GENT;
                             F.aleft = (int)GENT;
                         }
@@ -1462,39 +1504,45 @@ __global__ void __launch_bounds__(64) k_heavy(HArgs a)
                         hdr.w0 = (uint64_t)prev_base | (uint64_t)prev_size << 32;
                         hdr.w1 = 0;
                         POOL[F.abase] = hdr;
-                        pieces[np_++] = make_uint4((uint32_t)s, r0, plen, F.abase + 1);
+                        pieces[np_++] = make_uint4((uint32_t)sb, rk, plen, F.abase + 1);
                         prev_base = F.abase + 1;
                         prev_size = plen;
                         F.abase += plen + 1;
                         F.aleft -= (int)plen + 1;
                         rem -= plen;
-                        r0 += plen;
+                        rk += plen;
                     }
-                    seg_base[s] = prev_base;
-                    seg_cnt[s] = prev_size;
-                    F.bm[s >> 6] |= 1ull << (s & 63);
+                    seg_base[sb] = prev_base;
+                    seg_cnt[sb] = prev_size;
+                    F.bm[sb >> 6] |= 1ull << (sb & 63);
                 }
-                if (oom) {
+                if (badw) {  // a push outside the three buckets: not possible with positive penalties
+                    F.status = 1;
+                    F.result = ERR_INTERNAL;
+                } else if (oom) {
                     F.status = 1;
                     F.result = ERR_STACK;
                 }
-                F.n_piece = np_;
             }
             __syncthreads();
             if (F.status) break;
-            for (int u = 0; u < ncp; ++u) {
-                const int s = PSC[u];
-                const uint32_t r = cnt[s * HW + lane];
-                cnt[s * HW + lane] = (uint16_t)(r + 1);
-                int p = first_piece[s];
-                while (r >= pieces[p].y + pieces[p].z) ++p;
-                POOL[pieces[p].w + (r - pieces[p].y)] = PLOG[u];
+            const long long t_c2 = a.stats ? wall_clock64() : 0;
+            {
+                uint32_t q0 = (uint32_t)r0, q1 = (uint32_t)r1, q2 = (uint32_t)r2;
+                for (int u = 0; u < ncp; ++u) {
+                    const int sc = PSC[u];
+                    const uint32_t r = sc == tg0 ? q0++ : sc == tg1 ? q1++ : q2++;
+                    int p = first_piece[sc];
+                    while (r >= pieces[p].y + pieces[p].z) ++p;
+                    POOL[pieces[p].w + (r - pieces[p].y)] = PLOG[u];
+                }
             }
+            const int mxs = a.stats ? wave_max_i32(steps) : 0;
             __syncthreads();
+            const long long t_c3 = a.stats ? wall_clock64() : 0;
             // ---- bucket b: the committed start entries leave its top; the state moves on ----
             if (lane == 0) {
-                int taken = 0;
-                for (int j = 0; j <= last; ++j) taken += lcommit[j] > 0;
+                const int taken = __builtin_popcountll(posm);
                 const int from_seg = taken - (F.has_pend && taken > 0 ? 1 : 0);
                 if (taken > 0) F.has_pend = 0;
                 if (from_seg > 0) {
@@ -1511,34 +1559,32 @@ __global__ void __launch_bounds__(64) k_heavy(HArgs a)
                     }
                 }
                 // entries: every committed pop -1, every committed push +1, a pending child +1
-                long ne = F.n_entries + F.ne_delta;
-                const int lc = last >= 0 ? lcommit[last] : 0;
-                const bool partial = last >= 0 && lc < lsteps[last];
-                const bool trunc = last >= 0 && lc == lsteps[last] && lkind[last] == K_TRUNC;
-                if (partial) F.pend = SLOG_ALL[last * H_STEPS + lc];
-                if (partial || trunc) {
+                long ne = F.n_entries + sum_ne;
+                const int lfull = last >= 0 ? F.l_full : -1;
+                if (last >= 0 && (F.l_partial || lfull == K_TRUNC)) {
                     F.has_pend = 1;
                     ne += 1;
                 }
                 F.n_entries = (int)ne;
                 if (a.stats) {
-                    int cs = 0, mx = 0;
-                    for (int j = 0; j < HW; ++j) mx = max(mx, lsteps[j]);
-                    for (int j = 0; j <= last; ++j) cs += lcommit[j];
-                    atomicAdd(a.stats + 0, 1ull);
-                    atomicAdd(a.stats + 1, (unsigned long long)F.K);
-                    atomicAdd(a.stats + 2, (unsigned long long)(last + 1));
-                    atomicAdd(a.stats + 3, (unsigned long long)cs);
-                    atomicAdd(a.stats + 4, (unsigned long long)mx);
-                    atomicAdd(a.stats + 5, (unsigned long long)(t_commit - t_spec));
-                    atomicAdd(a.stats + 6, (unsigned long long)(wall_clock64() - t_commit));
+                    F.st[0] += 1;
+                    F.st[1] += (unsigned long long)K;
+                    F.st[2] += (unsigned long long)(last + 1);
+                    F.st[3] += sum_pops;
+                    F.st[4] += (unsigned long long)mxs;
+                    F.st[5] += (unsigned long long)(t_commit - t_spec);
+                    const long long t_end = wall_clock64();
+                    F.st[6] += (unsigned long long)(t_end - t_commit);
+                    F.st[7] += (unsigned long long)(t_c1 - t_commit);
+                    F.st[8] += (unsigned long long)(t_c2 - t_c1);
+                    F.st[9] += (unsigned long long)(t_c3 - t_c2);
+                    F.st[10] += (unsigned long long)(t_end - t_c3);
                 }
                 // a hit or a score error ends the last committed chain
-                const bool full = last >= 0 && lc == lsteps[last];
-                if (full && lkind[last] == K_ESCORE) {
+                if (lfull == K_ESCORE) {
                     F.status = 1;
                     F.result = ERR_SCORE;
-                } else if (full && lkind[last] == K_HIT) {
+                } else if (lfull == K_HIT) {
                     const Ent e = F.he;
                     const uint64_t k = F.hk, l = F.hl;
                     const int mm = e_mm(e), go = e_go(e), ge = e_ge(e);
@@ -1624,11 +1670,15 @@ __global__ void __launch_bounds__(64) k_heavy(HArgs a)
             a.out_pops[rd] = F.pops;
             if (TRACE) a.out_trace[rd] = F.trace;
             if (a.out_bytes) a.out_bytes[rd] = F.obytes;
-            if (F.n_own > 0) gran_push(a.ghead, a.gnext, OWN, 0, F.n_own);
-            F.n_own = 0;
+            // keep a few granules for the next read; the rest go back to the pool
+            if (F.n_own > H_KEEP) {
+                gran_push(a.ghead, a.gnext, OWN, H_KEEP, F.n_own);
+                F.n_own = H_KEEP;
+            }
         }
         __syncthreads();
     }
+    if (a.stats && lane < 16) atomicAdd(a.stats + lane, F.st[lane]);
 }
 
 // ---------------------------------------------------------------------------
@@ -1672,6 +1722,7 @@ struct galn_hip_index {
     uint32_t *d_pool_link = nullptr, *d_gnext = nullptr, *d_gbump = nullptr;
     unsigned long long *d_ghead = nullptr;
     uint32_t n_gran = 0;
+    std::atomic<int> n_readsets{0};  // read sets not yet freed (they use the stream and pool)
 };
 
 struct galn_hip_readset {
@@ -1705,8 +1756,10 @@ struct galn_hip_readset {
     uint8_t *d_scratch = nullptr;
     size_t scratch_bytes = 0;
     // host copies
-    std::vector<int32_t> h_qlen, h_nstacks, h_batch;
     std::vector<galn_search_opt_t> h_bopts;
+    // capacities of the device arrays (reads, query bytes, batches)
+    int cap_n = 0, cap_b = 0;
+    uint64_t cap_q = 0;
     // last search: one record per kernel launch, timed with events on the search stream
     double first_ms = 0, total_ms = 0;
     uint32_t last_flags = 0;
@@ -1782,7 +1835,7 @@ int ensure_pool(galn_hip_index *idx, uint64_t want)
         idx->n_gran = 0;
     }
     if (!idx->d_ghead) {
-        HIP_OK(hipMalloc(&idx->d_ghead, 8));
+        HIP_OK(hipMalloc(&idx->d_ghead, NSHARD * 8));
         HIP_OK(hipMalloc(&idx->d_gbump, 4));
     }
     HIP_OK(hipMalloc(&idx->d_pool_chk, want * gbytes));
@@ -1829,7 +1882,7 @@ int resident_slots(galn_hip_readset *rs, int block, bool trace, int waves_per_cu
 
 struct HLayout {
     uint64_t slot_bytes;
-    uint32_t rec_off, own_off, plog_off, pscore_off, slog_off, sinfo_off;
+    uint32_t rec_off, own_off, plog_off, slog_off, sinfo_off;
 };
 
 HLayout make_hlayout(int rec_cap, int own_cap)
@@ -1842,8 +1895,6 @@ HLayout make_hlayout(int rec_cap, int own_cap)
     off += align_up((size_t)own_cap * 4, 256);
     L.plog_off = (uint32_t)off;
     off += (size_t)HW * H_PUSH * sizeof(Ent);
-    L.pscore_off = (uint32_t)off;
-    off += align_up((size_t)HW * H_PUSH, 256);
     L.slog_off = (uint32_t)off;
     off += (size_t)HW * H_STEPS * sizeof(Ent);
     L.sinfo_off = (uint32_t)off;
@@ -1856,9 +1907,9 @@ HLayout make_hlayout(int rec_cap, int own_cap)
 size_t heavy_lds_bytes(int buckets, int wlen, int swlen)
 {
     size_t b = (sizeof(HShared) + 15) & ~(size_t)15;
-    b += (size_t)buckets * 16 + (size_t)buckets * HW * 2;
+    b += (size_t)buckets * 12;
     b = (b + 15) & ~(size_t)15;
-    b += (size_t)H_PIECES * 16 + (size_t)(wlen + swlen) * 8 + 7 * HW * 4;
+    b += (size_t)H_PIECES * 16 + (size_t)(wlen + swlen) * 8 + 2 * HW * 4 + (size_t)HW * H_PUSH;
     return b;
 }
 
@@ -1883,7 +1934,6 @@ KArgs base_args(galn_hip_readset *rs, uint32_t flags)
     a.batch = rs->d_batch;
     a.bopts = rs->d_bopts;
     a.nstacks = rs->d_nstacks;
-    a.seed_len = rs->seed_len;
     a.out_n = rs->d_out_n;
     a.out_off = rs->d_out_off;
     a.out_pops = rs->d_pops;
@@ -1942,7 +1992,7 @@ int run_tier(galn_hip_readset *rs, uint32_t flags, const Tier &t, const int32_t 
     a.heavy_n = rs->d_ctr + 3;
     if (persist) HIP_OK(hipMemsetAsync(rs->d_ctr + 2, 0, sizeof(int32_t), idx->stream));
     // an empty free stack (tag and granule all ones) and no granule handed out yet
-    HIP_OK(hipMemsetAsync(idx->d_ghead, 0xff, 8, idx->stream));
+    HIP_OK(hipMemsetAsync(idx->d_ghead, 0xff, NSHARD * 8, idx->stream));
     HIP_OK(hipMemsetAsync(idx->d_gbump, 0, 4, idx->stream));
     const int li = (int)rs->launches.size();
     if (li >= galn_hip_readset::MAX_LAUNCH) return fail(GALN_HIP_E_DEVICE, "too many launches in one search");
@@ -1994,7 +2044,6 @@ int run_heavy(galn_hip_readset *rs, uint32_t flags, const int32_t *list, int n, 
     a.batch = rs->d_batch;
     a.bopts = rs->d_bopts;
     a.nstacks = rs->d_nstacks;
-    a.seed_len = rs->seed_len;
     a.list = list;
     a.n_list = n;
     a.qnext = rs->d_ctr + 2;
@@ -2004,7 +2053,6 @@ int run_heavy(galn_hip_readset *rs, uint32_t flags, const int32_t *list, int n, 
     a.rec_off = L.rec_off;
     a.own_off = L.own_off;
     a.plog_off = L.plog_off;
-    a.pscore_off = L.pscore_off;
     a.slog_off = L.slog_off;
     a.sinfo_off = L.sinfo_off;
     a.own_cap = own_cap;
@@ -2028,9 +2076,9 @@ int run_heavy(galn_hip_readset *rs, uint32_t flags, const int32_t *list, int n, 
     a.buckets = buckets_of(rs);
     a.wlen = wlen;
     a.swlen = swlen;
-    a.stats = rs->d_stats;
+    a.stats = (flags & GALN_HIP_STATS) ? rs->d_stats : nullptr;
     HIP_OK(hipMemsetAsync(rs->d_ctr + 2, 0, sizeof(int32_t), idx->stream));
-    HIP_OK(hipMemsetAsync(idx->d_ghead, 0xff, 8, idx->stream));
+    HIP_OK(hipMemsetAsync(idx->d_ghead, 0xff, NSHARD * 8, idx->stream));
     HIP_OK(hipMemsetAsync(idx->d_gbump, 0, 4, idx->stream));
     const int li = (int)rs->launches.size();
     if (li >= galn_hip_readset::MAX_LAUNCH) return fail(GALN_HIP_E_DEVICE, "too many launches in one search");
@@ -2188,9 +2236,97 @@ int check_reads(const galn_hip_reads *r)
     if (r->n_reads > 0 && (!r->query || !r->qoff || !r->qlen || !r->max_diff || !r->batch))
         return fail(GALN_HIP_E_ARG, "reads: null array");
     if (r->n_batches < 1 || !r->bopts || !r->n_stacks) return fail(GALN_HIP_E_ARG, "reads: no batch options");
-    if (r->seed_len < 0) return fail(GALN_HIP_E_UNSUPPORTED, "reads: negative seed length");
+    for (int b = 0; b < r->n_batches; ++b)
+        if (r->bopts[b].seed_len < 0) return fail(GALN_HIP_E_UNSUPPORTED, "reads: negative seed length");
     for (int i = 0; i < r->n_reads; ++i)
         if (r->batch[i] < 0 || r->batch[i] >= r->n_batches) return fail(GALN_HIP_E_ARG, "reads: batch index out of range");
+    return 0;
+}
+
+// Grow the per-read and per-batch device arrays to hold n reads, qbytes query bytes and nb
+// batches (contents are not kept).
+int readset_reserve(galn_hip_readset *rs, int n, uint64_t qbytes, int nb)
+{
+    const bool grow_n = n > rs->cap_n, grow_q = qbytes > rs->cap_q, grow_b = nb > rs->cap_b;
+    if (!grow_n && !grow_q && !grow_b) return 0;
+    HIP_OK(hipStreamSynchronize(rs->idx->stream));
+    auto regrow = [](void **p, size_t bytes) __attribute__((always_inline)) -> bool {
+        if (*p) hipFree(*p);
+        *p = nullptr;
+        return hipMalloc(p, std::max<size_t>(bytes, 16)) == hipSuccess;
+    };
+    bool ok = true;
+    if (grow_q) {
+        const uint64_t c = std::max<uint64_t>(qbytes, rs->cap_q + rs->cap_q / 2);
+        ok = regrow((void **)&rs->d_query, c + 16);
+        rs->cap_q = ok ? c : 0;
+    }
+    if (ok && grow_b) {
+        const int c = std::max(nb, rs->cap_b + rs->cap_b / 2);
+        ok = regrow((void **)&rs->d_nstacks, (size_t)c * 4) && regrow((void **)&rs->d_bopts, (size_t)c * sizeof(galn_search_opt_t));
+        rs->cap_b = ok ? c : 0;
+    }
+    if (ok && grow_n) {
+        const size_t c = (size_t)std::max<int64_t>(n, (int64_t)rs->cap_n + rs->cap_n / 2);
+        ok = regrow((void **)&rs->d_qoff, c * 8) && regrow((void **)&rs->d_qlen, c * 4) &&
+             regrow((void **)&rs->d_maxdiff, c * 4) && regrow((void **)&rs->d_batch, c * 4) &&
+             regrow((void **)&rs->d_sorted, c * 4) && regrow((void **)&rs->d_out_n, c * 4) &&
+             regrow((void **)&rs->d_out_off, c * 8) && regrow((void **)&rs->d_pops, c * 8) &&
+             regrow((void **)&rs->d_trace, c * 8) && regrow((void **)&rs->d_bytes, c * 8) &&
+             regrow((void **)&rs->d_fail[0], c * 4) && regrow((void **)&rs->d_fail[1], c * 4) &&
+             regrow((void **)&rs->d_heavy, c * 4);
+        rs->cap_n = ok ? (int)c : 0;
+        if (ok && rs->rec_cap < 2 * c) {
+            rs->rec_cap = std::max<uint64_t>(2 * c, 1 << 16);
+            ok = regrow((void **)&rs->d_rec, rs->rec_cap * sizeof(galn_aln_rec_t));
+            if (!ok) rs->rec_cap = 0;
+        }
+    }
+    if (!ok) return fail(GALN_HIP_E_NOMEM, "readset: device allocation failed");
+    return 0;
+}
+
+// Copy a read set (already checked) into rs's device arrays.
+int readset_fill(galn_hip_readset *rs, const galn_hip_reads *reads)
+{
+    const int n = reads->n_reads;
+    uint64_t qbytes = 0;
+    int max_len = 1;
+    for (int i = 0; i < n; ++i) {
+        qbytes = std::max<uint64_t>(qbytes, reads->qoff[i] + (uint64_t)std::max(reads->qlen[i], 0));
+        max_len = std::max(max_len, reads->qlen[i]);
+    }
+    int r = readset_reserve(rs, std::max(n, 1), qbytes, reads->n_batches);
+    if (r) return r;
+    rs->n = n;
+    rs->n_batches = reads->n_batches;
+    rs->max_len = std::min(max_len, MAXLEN);
+    rs->qbytes = qbytes;
+    // seed widths are kept for the longest seed that is shorter than some read
+    rs->seed_len = 0;
+    for (int b = 0; b < reads->n_batches; ++b)
+        if (reads->bopts[b].seed_len < rs->max_len) rs->seed_len = std::max(rs->seed_len, reads->bopts[b].seed_len);
+    rs->max_stacks = 1;
+    for (int b = 0; b < reads->n_batches; ++b) rs->max_stacks = std::max(rs->max_stacks, reads->n_stacks[b]);
+    rs->h_bopts.assign(reads->bopts, reads->bopts + reads->n_batches);
+    rs->max_entries = 0;
+    for (const auto &o : rs->h_bopts) rs->max_entries = std::max(rs->max_entries, std::max(o.max_entries, 0));
+    // claim order of the batched form: longest reads first (their searches are the longest)
+    std::vector<int32_t> order(n);
+    std::iota(order.begin(), order.end(), 0);
+    std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return reads->qlen[x] > reads->qlen[y]; });
+    hipStream_t s = rs->idx->stream;
+    if (hipMemcpyAsync(rs->d_query, reads->query, qbytes, hipMemcpyHostToDevice, s) != hipSuccess ||
+        hipMemcpyAsync(rs->d_qoff, reads->qoff, (size_t)n * 8, hipMemcpyHostToDevice, s) != hipSuccess ||
+        hipMemcpyAsync(rs->d_qlen, reads->qlen, (size_t)n * 4, hipMemcpyHostToDevice, s) != hipSuccess ||
+        hipMemcpyAsync(rs->d_maxdiff, reads->max_diff, (size_t)n * 4, hipMemcpyHostToDevice, s) != hipSuccess ||
+        hipMemcpyAsync(rs->d_batch, reads->batch, (size_t)n * 4, hipMemcpyHostToDevice, s) != hipSuccess ||
+        hipMemcpyAsync(rs->d_nstacks, reads->n_stacks, (size_t)reads->n_batches * 4, hipMemcpyHostToDevice, s) != hipSuccess ||
+        hipMemcpyAsync(rs->d_bopts, reads->bopts, (size_t)reads->n_batches * sizeof(galn_search_opt_t),
+                       hipMemcpyHostToDevice, s) != hipSuccess ||
+        hipMemcpyAsync(rs->d_sorted, order.data(), (size_t)n * 4, hipMemcpyHostToDevice, s) != hipSuccess ||
+        hipStreamSynchronize(s) != hipSuccess)
+        return fail(GALN_HIP_E_DEVICE, "readset: copy failed");
     return 0;
 }
 
@@ -2211,6 +2347,15 @@ void galn_hip_get_limits(galn_hip_limits_t *lim)
 }
 
 const char *galn_hip_last_error(void) { return g_err; }
+
+}  // extern "C"
+
+// Used by the serving loop (galn_server.cpp), not exported: this thread's error text, and
+// the count of handles that keep an index open.
+__attribute__((visibility("hidden"))) void galn_hip_set_error(const char *msg) { snprintf(g_err, sizeof(g_err), "%s", msg); }
+__attribute__((visibility("hidden"))) void galn_hip_index_hold(galn_hip_index *idx, int delta) { idx->n_readsets += delta; }
+
+extern "C" {
 
 int galn_hip_index_open(const uint32_t *bwt, const galn_bwt_meta_t *meta, int device, galn_hip_index **out)
 {
@@ -2256,6 +2401,9 @@ int galn_hip_index_open(const uint32_t *bwt, const galn_bwt_meta_t *meta, int de
 int galn_hip_index_close(galn_hip_index *idx)
 {
     if (!idx) return 0;
+    if (idx->n_readsets.load() > 0)
+        return fail(GALN_HIP_E_BUSY, "index_close: %d read sets (or servers) of this index are still open",
+                    idx->n_readsets.load());
     hipSetDevice(idx->device);
     if (idx->stream) hipStreamSynchronize(idx->stream);
     hipFree(idx->d_bwt);
@@ -2283,6 +2431,7 @@ int galn_hip_readset_free(galn_hip_readset *rs)
     for (auto &e : rs->ev)
         if (e) hipEventDestroy(e);
     for (auto &e : rs->lev) hipEventDestroy(e);
+    rs->idx->n_readsets.fetch_sub(1);
     delete rs;
     return 0;
 }
@@ -2296,39 +2445,9 @@ int galn_hip_readset_upload(galn_hip_index *idx, const galn_hip_reads *reads, ga
     HIP_OK(hipSetDevice(idx->device));
     galn_hip_readset *rs = new galn_hip_readset;
     rs->idx = idx;
-    const int n = reads->n_reads;
-    rs->n = n;
-    rs->n_batches = reads->n_batches;
-    rs->seed_len = reads->seed_len;
-    uint64_t qbytes = 0;
-    int max_len = 1;
-    for (int i = 0; i < n; ++i) {
-        qbytes = std::max<uint64_t>(qbytes, reads->qoff[i] + (uint64_t)std::max(reads->qlen[i], 0));
-        max_len = std::max(max_len, reads->qlen[i]);
-    }
-    rs->max_len = std::min(max_len, MAXLEN);
-    rs->qbytes = qbytes;
-    rs->max_stacks = 1;
-    for (int b = 0; b < reads->n_batches; ++b) rs->max_stacks = std::max(rs->max_stacks, reads->n_stacks[b]);
-    rs->h_qlen.assign(reads->qlen, reads->qlen + n);
-    rs->h_batch.assign(reads->batch, reads->batch + n);
-    rs->h_nstacks.assign(reads->n_stacks, reads->n_stacks + reads->n_batches);
-    rs->h_bopts.assign(reads->bopts, reads->bopts + reads->n_batches);
-    rs->max_entries = 0;
-    for (const auto &o : rs->h_bopts) rs->max_entries = std::max(rs->max_entries, std::max(o.max_entries, 0));
-    const size_t nn = (size_t)std::max(n, 1);
-    auto M = [&](void **p, size_t b) __attribute__((always_inline)) -> bool { return hipMalloc(p, std::max<size_t>(b, 16)) == hipSuccess; };
-    rs->rec_cap = std::max<uint64_t>((uint64_t)n * 2, 1 << 16);
-    bool ok = M((void **)&rs->d_query, qbytes + 16) && M((void **)&rs->d_qoff, nn * 8) && M((void **)&rs->d_qlen, nn * 4) &&
-              M((void **)&rs->d_maxdiff, nn * 4) && M((void **)&rs->d_batch, nn * 4) &&
-              M((void **)&rs->d_nstacks, reads->n_batches * 4) &&
-              M((void **)&rs->d_bopts, reads->n_batches * sizeof(galn_search_opt_t)) && M((void **)&rs->d_sorted, nn * 4) &&
-              M((void **)&rs->d_out_n, nn * 4) && M((void **)&rs->d_out_off, nn * 8) && M((void **)&rs->d_pops, nn * 8) &&
-              M((void **)&rs->d_trace, nn * 8) && M((void **)&rs->d_bytes, nn * 8) &&
-              M((void **)&rs->d_rec, rs->rec_cap * sizeof(galn_aln_rec_t)) &&
-              M((void **)&rs->d_top, 8) && M((void **)&rs->d_fail[0], nn * 4) && M((void **)&rs->d_fail[1], nn * 4) &&
-              M((void **)&rs->d_ctr, 8 * sizeof(int32_t)) && M((void **)&rs->d_heavy, nn * 4) &&
-              M((void **)&rs->d_stats, 16 * 8) &&
+    idx->n_readsets.fetch_add(1);
+    auto M = [&](void **p, size_t b) __attribute__((always_inline)) -> bool { return hipMalloc(p, b) == hipSuccess; };
+    bool ok = M((void **)&rs->d_top, 8) && M((void **)&rs->d_ctr, 8 * sizeof(int32_t)) && M((void **)&rs->d_stats, 16 * 8) &&
               M((void **)&rs->d_algo, sizeof(unsigned long long) * galn_hip_readset::MAX_LAUNCH);
     if (!ok) {
         galn_hip_readset_free(rs);
@@ -2339,26 +2458,21 @@ int galn_hip_readset_upload(galn_hip_index *idx, const galn_hip_reads *reads, ga
             galn_hip_readset_free(rs);
             return fail(GALN_HIP_E_DEVICE, "hipEventCreate failed");
         }
-    // claim order of the batched form: longest reads first (their searches are the longest)
-    std::vector<int32_t> order(n);
-    std::iota(order.begin(), order.end(), 0);
-    std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return reads->qlen[x] > reads->qlen[y]; });
-    hipStream_t s = idx->stream;
-    if (hipMemcpyAsync(rs->d_query, reads->query, qbytes, hipMemcpyHostToDevice, s) != hipSuccess ||
-        hipMemcpyAsync(rs->d_qoff, reads->qoff, (size_t)n * 8, hipMemcpyHostToDevice, s) != hipSuccess ||
-        hipMemcpyAsync(rs->d_qlen, reads->qlen, (size_t)n * 4, hipMemcpyHostToDevice, s) != hipSuccess ||
-        hipMemcpyAsync(rs->d_maxdiff, reads->max_diff, (size_t)n * 4, hipMemcpyHostToDevice, s) != hipSuccess ||
-        hipMemcpyAsync(rs->d_batch, reads->batch, (size_t)n * 4, hipMemcpyHostToDevice, s) != hipSuccess ||
-        hipMemcpyAsync(rs->d_nstacks, reads->n_stacks, (size_t)reads->n_batches * 4, hipMemcpyHostToDevice, s) != hipSuccess ||
-        hipMemcpyAsync(rs->d_bopts, reads->bopts, (size_t)reads->n_batches * sizeof(galn_search_opt_t),
-                       hipMemcpyHostToDevice, s) != hipSuccess ||
-        hipMemcpyAsync(rs->d_sorted, order.data(), (size_t)n * 4, hipMemcpyHostToDevice, s) != hipSuccess ||
-        hipStreamSynchronize(s) != hipSuccess) {
+    if ((r = readset_fill(rs, reads))) {
         galn_hip_readset_free(rs);
-        return fail(GALN_HIP_E_DEVICE, "readset_upload: copy failed");
+        return r;
     }
     *out = rs;
     return 0;
+}
+
+int galn_hip_readset_reload(galn_hip_readset *rs, const galn_hip_reads *reads)
+{
+    if (!rs) return fail(GALN_HIP_E_ARG, "readset_reload: null handle");
+    int r = check_reads(reads);
+    if (r) return r;
+    HIP_OK(hipSetDevice(rs->idx->device));
+    return readset_fill(rs, reads);
 }
 
 int galn_hip_readset_search(galn_hip_index *idx, galn_hip_readset *rs, int form, uint32_t flags,
@@ -2422,10 +2536,10 @@ int galn_hip_readset_kernel_ms(const galn_hip_readset *rs, double *first_tier_ms
     return 0;
 }
 
-int galn_hip_readset_engine_stats(galn_hip_readset *rs, uint64_t out[8])
+int galn_hip_readset_engine_stats(galn_hip_readset *rs, uint64_t out[16])
 {
     if (!rs || !out) return fail(GALN_HIP_E_ARG, "readset_engine_stats: null argument");
-    HIP_OK(hipMemcpy(out, rs->d_stats, 8 * 8, hipMemcpyDeviceToHost));
+    HIP_OK(hipMemcpy(out, rs->d_stats, 16 * 8, hipMemcpyDeviceToHost));
     return 0;
 }
 

@@ -582,7 +582,6 @@ extern "C" int galn_aln_main(int argc, char **argv)
     R.n_batches = nb;
     R.bopts = bopts.data();
     R.n_stacks = nstacks.data();
-    R.seed_len = A.opt.seed_len;
     std::vector<int32_t> n_aln(std::max(n, 1));
     std::vector<uint64_t> rec_off(std::max(n, 1)), pops(std::max(n, 1)), trace(std::max(n, 1));
     std::vector<galn_aln_rec_t> rec((size_t)std::max(n, 1) * 2);

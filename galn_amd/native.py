@@ -7,6 +7,7 @@ from __future__ import annotations
 
 import ctypes as C
 import os
+import weakref
 
 import numpy as np
 
@@ -18,6 +19,7 @@ GALN_BIN = os.path.join(PKG, "bin", "galn")
 
 GALN_HIP_TRACE = 0x1
 GALN_HIP_NOFUSE = 0x2
+GALN_HIP_STATS = 0x4
 FORM_PLAIN, FORM_BATCHED = 0, 1
 E_RECCAP = -5
 
@@ -25,8 +27,9 @@ HIP_SYMBOLS = (
     "galn_hip_abi_version", "galn_hip_get_limits", "galn_hip_last_error", "galn_hip_index_open",
     "galn_hip_index_close", "galn_hip_index_device", "galn_hip_search_plain", "galn_hip_search_batched",
     "galn_hip_readset_upload", "galn_hip_readset_free", "galn_hip_readset_search", "galn_hip_readset_results",
-    "galn_hip_readset_kernel_ms", "galn_hip_readset_launches", "galn_hip_readset_engine_stats",
-    "galn_hip_gather_ceiling",
+    "galn_hip_readset_reload", "galn_hip_readset_kernel_ms", "galn_hip_readset_launches",
+    "galn_hip_readset_engine_stats", "galn_hip_server_open", "galn_hip_server_submit", "galn_hip_server_wait",
+    "galn_hip_server_result", "galn_hip_server_stats", "galn_hip_server_close", "galn_hip_gather_ceiling",
 )
 HOST_SYMBOLS = (
     "galn_gap_opt_default", "galn_parse_aln_args", "galn_cal_maxdiff", "galn_batch_opt", "galn_read_maxdiff",
@@ -53,7 +56,7 @@ class AlnRec(C.Structure):
 class Reads(C.Structure):
     _fields_ = [("n_reads", C.c_int32), ("query", C.c_void_p), ("qoff", C.c_void_p), ("qlen", C.c_void_p),
                 ("max_diff", C.c_void_p), ("batch", C.c_void_p), ("n_batches", C.c_int32), ("bopts", C.c_void_p),
-                ("n_stacks", C.c_void_p), ("seed_len", C.c_int32)]
+                ("n_stacks", C.c_void_p)]
 
 
 class Out(C.Structure):
@@ -69,6 +72,11 @@ class Limits(C.Structure):
 class Tune(C.Structure):
     _fields_ = [(n, C.c_int32) for n in ("block_threads", "waves_per_cu", "chunks_per_lane", "rec_per_lane",
                                          "sort_by_cost", "heavy_pops")]
+
+
+class ServerCfg(C.Structure):
+    _fields_ = [("max_launch_reads", C.c_int32), ("quantum_reads", C.c_int32), ("max_queued_bytes", C.c_uint64),
+                ("flags", C.c_uint32), ("form", C.c_int32)]
 
 
 class GapOpt(C.Structure):
@@ -105,6 +113,13 @@ def hip_lib():
         L.galn_hip_index_close.argtypes = [P]
         L.galn_hip_readset_upload.argtypes = [P, C.POINTER(Reads), C.POINTER(P)]
         L.galn_hip_readset_free.argtypes = [P]
+        L.galn_hip_readset_reload.argtypes = [P, C.POINTER(Reads)]
+        L.galn_hip_server_open.argtypes = [P, C.POINTER(ServerCfg), C.POINTER(P)]
+        L.galn_hip_server_submit.argtypes = [P, C.POINTER(Reads), C.POINTER(C.c_uint64)]
+        L.galn_hip_server_wait.argtypes = [P, C.c_uint64, C.c_int]
+        L.galn_hip_server_result.argtypes = [P, C.c_uint64, C.POINTER(Out), C.POINTER(C.c_double)]
+        L.galn_hip_server_stats.argtypes = [P, P]
+        L.galn_hip_server_close.argtypes = [P]
         L.galn_hip_readset_search.argtypes = [P, P, C.c_int, C.c_uint32, C.POINTER(Tune)]
         L.galn_hip_readset_results.argtypes = [P, C.POINTER(Out)]
         L.galn_hip_readset_kernel_ms.argtypes = [P, C.POINTER(C.c_double), C.POINTER(C.c_double)]
@@ -268,9 +283,11 @@ def read_fastq(path: str) -> dict:
 
 
 class Index:
-    """A .bwt body resident in one device's HBM."""
+    """A .bwt body resident in one device's HBM. Closing it first closes the read sets and
+    servers opened on it (the library refuses to close an index that still has them)."""
 
     def __init__(self, words: np.ndarray, meta: BwtMeta | dict, device: int = 0):
+        self._children = weakref.WeakSet()
         self.meta = meta if isinstance(meta, BwtMeta) else meta_from_dict(meta)
         self._words = np.ascontiguousarray(words, dtype=np.uint32)
         h = C.c_void_p()
@@ -282,7 +299,11 @@ class Index:
 
     def close(self):
         if self.h:
-            hip_lib().galn_hip_index_close(self.h)
+            for c in list(self._children):
+                c.close()
+            rc = hip_lib().galn_hip_index_close(self.h)
+            if rc != 0:
+                raise GalnError(f"index_close failed ({rc}): {last_error()}")
             self.h = None
 
     def __enter__(self):
@@ -309,30 +330,43 @@ class Results:
         return self.rec[o:o + n]
 
 
+def make_reads(query, qoff, qlen, max_diff, batch, bopts, n_stacks) -> tuple[Reads, list]:
+    """A Reads structure over contiguous copies of the arrays (returned with it: keep them
+    alive while the structure is used)."""
+    keep = [np.ascontiguousarray(query, dtype=np.uint8), np.ascontiguousarray(qoff, dtype=np.uint64),
+            np.ascontiguousarray(qlen, dtype=np.int32), np.ascontiguousarray(max_diff, dtype=np.int32),
+            np.ascontiguousarray(batch, dtype=np.int32), np.ascontiguousarray(n_stacks, dtype=np.int32), bopts]
+    q, qo, ql, md, bi, ns, _ = keep
+    R = Reads()
+    R.n_reads = int(ql.size)
+    R.query, R.qoff, R.qlen, R.max_diff, R.batch = _p(q), _p(qo), _p(ql), _p(md), _p(bi)
+    R.n_batches = int(ns.size)
+    R.bopts = C.cast(bopts, C.c_void_p)
+    R.n_stacks = _p(ns)
+    return R, keep
+
+
 class ReadSet:
     """Reads resident on the device (galn_hip_readset_*)."""
 
-    def __init__(self, index: Index, query, qoff, qlen, max_diff, batch, bopts, n_stacks, seed_len: int):
+    def __init__(self, index: Index, query, qoff, qlen, max_diff, batch, bopts, n_stacks):
         self.index = index
-        self._arrays = [np.ascontiguousarray(query, dtype=np.uint8), np.ascontiguousarray(qoff, dtype=np.uint64),
-                        np.ascontiguousarray(qlen, dtype=np.int32), np.ascontiguousarray(max_diff, dtype=np.int32),
-                        np.ascontiguousarray(batch, dtype=np.int32), np.ascontiguousarray(n_stacks, dtype=np.int32)]
-        q, qo, ql, md, bi, ns = self._arrays
-        self.n = int(ql.size)
-        R = Reads()
-        R.n_reads = self.n
-        R.query, R.qoff, R.qlen, R.max_diff, R.batch = _p(q), _p(qo), _p(ql), _p(md), _p(bi)
-        R.n_batches = int(ns.size)
-        R.bopts = C.cast(bopts, C.c_void_p)
-        R.n_stacks = _p(ns)
-        R.seed_len = int(seed_len)
-        self._bopts = bopts
+        R, keep = make_reads(query, qoff, qlen, max_diff, batch, bopts, n_stacks)
+        self.n = int(R.n_reads)
         h = C.c_void_p()
         rc = hip_lib().galn_hip_readset_upload(index.h, C.byref(R), C.byref(h))
         if rc != 0:
             raise GalnError(f"readset_upload failed ({rc}): {last_error()}")
         self.h = h
-        self._arrays = None
+        index._children.add(self)
+
+    def reload(self, query, qoff, qlen, max_diff, batch, bopts, n_stacks) -> None:
+        """Replace the reads, reusing the device arrays when they are large enough."""
+        R, keep = make_reads(query, qoff, qlen, max_diff, batch, bopts, n_stacks)
+        rc = hip_lib().galn_hip_readset_reload(self.h, C.byref(R))
+        if rc != 0:
+            raise GalnError(f"readset_reload failed ({rc}): {last_error()}")
+        self.n = int(R.n_reads)
 
     def search(self, form: int = FORM_BATCHED, flags: int = 0, tune: Tune | None = None) -> None:
         rc = hip_lib().galn_hip_readset_search(self.index.h, self.h, int(form), int(flags),
@@ -352,10 +386,10 @@ class ReadSet:
 
     def engine_stats(self) -> dict:
         """Counters of the wave-per-read engine in the last search."""
-        v = np.zeros(8, np.uint64)
+        v = np.zeros(16, np.uint64)
         hip_lib().galn_hip_readset_engine_stats(self.h, _p(v))
         keys = ("rounds", "chains", "chains_committed", "steps_committed", "critical_steps", "spec_ticks",
-                "commit_ticks")
+                "commit_ticks", "plan_ticks", "alloc_ticks", "place_ticks", "update_ticks", "granules")
         return {k: int(v[i]) for i, k in enumerate(keys)}
 
     def kernel_ms(self) -> tuple[float, float]:
@@ -392,6 +426,90 @@ class ReadSet:
         if self.h:
             hip_lib().galn_hip_readset_free(self.h)
             self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Server:
+    """The serving loop (galn_hip_server_*): the index stays resident; requests are queued
+    and searched in shared launches by a worker thread."""
+
+    def __init__(self, index: Index, max_launch_reads: int = 0, quantum_reads: int = 0, max_queued_bytes: int = 0,
+                 flags: int = 0, form: int = FORM_BATCHED):
+        cfg = ServerCfg()
+        cfg.max_launch_reads, cfg.quantum_reads = int(max_launch_reads), int(quantum_reads)
+        cfg.max_queued_bytes, cfg.flags, cfg.form = int(max_queued_bytes), int(flags), int(form)
+        h = C.c_void_p()
+        rc = hip_lib().galn_hip_server_open(index.h, C.byref(cfg), C.byref(h))
+        if rc != 0:
+            raise GalnError(f"server_open failed ({rc}): {last_error()}")
+        self.h = h
+        self.index = index
+        index._children.add(self)
+        self.n_of: dict[int, int] = {}
+
+    def submit(self, query, qoff, qlen, max_diff, batch, bopts, n_stacks) -> int | None:
+        """The request id, or None when the admission limit is reached (GALN_HIP_E_BUSY)."""
+        R, keep = make_reads(query, qoff, qlen, max_diff, batch, bopts, n_stacks)
+        rid = C.c_uint64()
+        rc = hip_lib().galn_hip_server_submit(self.h, C.byref(R), C.byref(rid))
+        if rc == -6:
+            return None
+        if rc != 0:
+            raise GalnError(f"server_submit failed ({rc}): {last_error()}")
+        self.n_of[rid.value] = int(R.n_reads)
+        return rid.value
+
+    def wait(self, rid: int, timeout_ms: int = -1) -> bool:
+        rc = hip_lib().galn_hip_server_wait(self.h, int(rid), int(timeout_ms))
+        if rc < 0:
+            raise GalnError(f"request {rid} failed ({rc}): {last_error()}")
+        return rc == 1
+
+    def result(self, rid: int, want_trace: bool = False) -> tuple[Results, float]:
+        """(results, latency in ms) of a finished request; releases it."""
+        n = self.n_of[rid]
+        n_aln = np.zeros(max(n, 1), np.int32)
+        rec_off = np.zeros(max(n, 1), np.uint64)
+        pops = np.zeros(max(n, 1), np.uint64)
+        trace = np.zeros(max(n, 1), np.uint64) if want_trace else None
+        cap = max(2 * n, 1)
+        lat = C.c_double()
+        while True:
+            rec = np.zeros((cap, 3), np.uint64)
+            O = Out()
+            O.n_aln, O.rec_off, O.rec, O.rec_cap = _p(n_aln), _p(rec_off), _p(rec), cap
+            O.n_pops = _p(pops)
+            O.trace = _p(trace) if trace is not None else None
+            rc = hip_lib().galn_hip_server_result(self.h, int(rid), C.byref(O), C.byref(lat))
+            if rc == E_RECCAP:
+                cap = max(int(O.n_rec), 1)
+                continue
+            if rc != 0:
+                raise GalnError(f"server_result failed ({rc}): {last_error()}")
+            del self.n_of[rid]
+            return Results(n_aln[:n], rec_off[:n], rec[:int(O.n_rec)], pops[:n],
+                           trace[:n] if trace is not None else None, None), lat.value
+
+    def stats(self) -> dict:
+        v = np.zeros(8, np.float64)
+        hip_lib().galn_hip_server_stats(self.h, _p(v))
+        return {"launches": int(v[0]), "reads": int(v[1]), "requests": int(v[2]), "launch_ms": float(v[3])}
+
+    def close(self):
+        if self.h:
+            hip_lib().galn_hip_server_close(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
 
     def __del__(self):
         try:

@@ -59,7 +59,7 @@ class Workload:
 
     def readset(self, index: "nv.Index") -> "nv.ReadSet":
         return nv.ReadSet(index, self.query, self.qoff, self.lens.astype(np.int32), self.maxdiff, self.batch,
-                          self.bopts, self.nstacks, self.seed_len)
+                          self.bopts, self.nstacks)
 
     def fastq(self, path: str, start: int = 0, count: int | None = None) -> str:
         end = self.n if count is None else min(self.n, start + count)
@@ -68,7 +68,12 @@ class Workload:
 
 
 def _index(ref: np.ndarray, prefix: str, device: str) -> tuple[dict, np.ndarray]:
-    ix.build_index(ref.tobytes(), prefix, device=device)
+    """Build <prefix>.bwt, or reuse one an earlier call finished (the name carries the
+    generator's size and seed; <prefix>.done is written after a complete build)."""
+    done = prefix + ".done"
+    if not (os.path.exists(done) and os.path.exists(prefix + ".bwt")):
+        ix.build_index(ref.tobytes(), prefix, device=device)
+        open(done, "w").close()
     return ix.read_bwt(prefix + ".bwt")
 
 

@@ -28,7 +28,7 @@
 extern "C" {
 #endif
 
-#define GALN_HIP_ABI_VERSION 1
+#define GALN_HIP_ABI_VERSION 2
 
 /* ---- library return codes ---- */
 #define GALN_HIP_OK             0
@@ -37,7 +37,8 @@ extern "C" {
 #define GALN_HIP_E_NOMEM      (-3)   /* device memory exhausted */
 #define GALN_HIP_E_UNSUPPORTED (-4)  /* a read or option outside the kernel's limits */
 #define GALN_HIP_E_RECCAP     (-5)   /* caller's record array too small: out->n_rec says how many */
-#define GALN_HIP_E_BUSY       (-6)   /* serving: admission limit reached */
+#define GALN_HIP_E_BUSY       (-6)   /* serving: admission limit reached, or result not ready;
+                                        index_close: read sets or servers still open */
 
 /* ---- per-read result codes (out->n_aln[i] < 0), as src/bwtgap_hd.h:41-43 ---- */
 #define GALN_READ_ERR_SCORE   (-3)   /* a push with score outside [0, n_stacks): final */
@@ -46,6 +47,7 @@ extern "C" {
 /* search flags */
 #define GALN_HIP_TRACE   0x1u        /* compute the per-read trace hash (pop-order check) */
 #define GALN_HIP_NOFUSE  0x2u        /* never keep the exact child in registers (testing) */
+#define GALN_HIP_STATS   0x4u        /* count the wave-per-read engine's rounds and clocks */
 
 /* Index descriptor: field order of bwtgpu_meta_t, src/bwtsearch.h:26-31. */
 typedef struct {
@@ -84,9 +86,9 @@ typedef struct {
     const int32_t *max_diff;   /* [n_reads] per-read max_diff (aln_read_maxdiff) */
     const int32_t *batch;      /* [n_reads] index into bopts / n_stacks */
     int32_t n_batches;
-    const galn_search_opt_t *bopts;   /* [n_batches] (aln_batch_opt) */
+    const galn_search_opt_t *bopts;   /* [n_batches] (aln_batch_opt; seed_len is the -l value,
+                                         and a read is seeded when seed_len < its length) */
     const int32_t *n_stacks;          /* [n_batches] */
-    int32_t seed_len;          /* the -l value of the run (gap_opt_t.seed_len) */
 } galn_hip_reads;
 
 /* Results, caller-owned. Records of read i are rec[rec_off[i] .. rec_off[i] + n_aln[i]). */
@@ -122,6 +124,19 @@ typedef struct {
                                   (0: default, -1: never) */
 } galn_hip_tune_t;
 
+/* Serving loop configuration (0 = default). */
+typedef struct {
+    int32_t max_launch_reads;  /* reads searched together in one launch (default 262144) */
+    int32_t quantum_reads;     /* reads a request gets per launch before the next request's
+                                  reads are taken (default 16384) */
+    uint64_t max_queued_bytes; /* admission: device bytes of the requests not yet collected
+                                  (query bytes + 176 per read: inputs, outputs and two
+                                  record slots); a submit beyond it returns
+                                  GALN_HIP_E_BUSY (default 4 GiB) */
+    uint32_t flags;            /* search flags of every launch */
+    int32_t form;              /* 1 = batched (default), 0 = plain */
+} galn_hip_server_cfg_t;
+
 typedef struct galn_hip_index galn_hip_index;
 typedef struct galn_hip_readset galn_hip_readset;
 typedef struct galn_hip_server galn_hip_server;
@@ -134,6 +149,7 @@ const char *galn_hip_last_error(void);
 /* Upload a .bwt body (bwt_size words, as load_bwt returns it) to `device`. */
 int galn_hip_index_open(const uint32_t *bwt, const galn_bwt_meta_t *meta, int device,
                         galn_hip_index **out);
+/* Returns GALN_HIP_E_BUSY (and keeps the index) while read sets or servers of it are open. */
 int galn_hip_index_close(galn_hip_index *idx);
 int galn_hip_index_device(const galn_hip_index *idx);
 
@@ -150,6 +166,9 @@ int galn_hip_search_batched(galn_hip_index *idx, const galn_hip_reads *reads, ga
 /* Device-resident reads: upload once, search many times (benchmarks, serving). */
 int galn_hip_readset_upload(galn_hip_index *idx, const galn_hip_reads *reads, galn_hip_readset **out);
 int galn_hip_readset_free(galn_hip_readset *rs);
+/* Replace the reads of a read set, reusing its device arrays when they are large enough
+ * (the serving loop's launches). */
+int galn_hip_readset_reload(galn_hip_readset *rs, const galn_hip_reads *reads);
 /* form: 0 = plain, 1 = batched. Results stay on the device until galn_hip_readset_results. */
 int galn_hip_readset_search(galn_hip_index *idx, galn_hip_readset *rs, int form, uint32_t flags,
                             const galn_hip_tune_t *tune);
@@ -161,10 +180,34 @@ int galn_hip_readset_kernel_ms(const galn_hip_readset *rs, double *first_tier_ms
  * search stream) and the algorithmic index bytes of the reads it finished. Returns the number
  * of launches (fills at most `cap`). */
 int galn_hip_readset_launches(const galn_hip_readset *rs, int cap, int32_t *tier, double *ms, uint64_t *algo_bytes);
-/* Counters of the wave-per-read engine in the last search: rounds, chains started, chains
- * committed, steps committed, longest chain per round (summed), clock ticks in speculation
- * and in commits (summed over waves), 0. */
-int galn_hip_readset_engine_stats(galn_hip_readset *rs, uint64_t out[8]);
+/* Counters of the wave-per-read engine in the last search made with GALN_HIP_STATS: rounds,
+ * chains started, chains committed, steps committed, longest chain per round (summed), then
+ * clock ticks (100 MHz, summed over waves) in speculation, in commits, and in the commits'
+ * parts: plan, piece allocation, placement, state update; granules taken; 0 x 4. */
+int galn_hip_readset_engine_stats(galn_hip_readset *rs, uint64_t out[16]);
+
+/* Serving: the index stays resident, requests (each a galn_hip_reads with its own options)
+ * are queued by submit and searched by a worker thread in launches of up to
+ * max_launch_reads reads; each launch takes up to quantum_reads reads of every waiting
+ * request in arrival order, so a small request is not held behind a large one, and reads
+ * of different requests share a launch (continuous batching). A read's records are the
+ * same whether it is searched alone or in a mix. While a server is open, the index handle
+ * belongs to it: make no other call on that index until galn_hip_server_close.
+ * submit copies the request (the caller's arrays may be reused on return) and returns
+ * its id; wait blocks up to timeout_ms (0 = poll, < 0 = forever) and returns 1 when the
+ * request is finished, 0 if not yet; result copies the records out like
+ * galn_hip_readset_results, gives the latency (submit to finish, ms) and releases the
+ * request (unless it returns GALN_HIP_E_RECCAP). A request whose search failed reports
+ * the error from wait and result. Thread-safe. */
+int galn_hip_server_open(galn_hip_index *idx, const galn_hip_server_cfg_t *cfg, galn_hip_server **out);
+int galn_hip_server_submit(galn_hip_server *srv, const galn_hip_reads *reads, uint64_t *id);
+int galn_hip_server_wait(galn_hip_server *srv, uint64_t id, int timeout_ms);
+int galn_hip_server_result(galn_hip_server *srv, uint64_t id, galn_hip_out *out, double *latency_ms);
+/* Counters since open: launches, reads searched, requests finished, summed launch ms (host
+ * clock around each launch), 0, 0, 0, 0. */
+int galn_hip_server_stats(galn_hip_server *srv, double out[8]);
+/* Finishes the requests already submitted, then stops the worker. */
+int galn_hip_server_close(galn_hip_server *srv);
 
 /* Random-access ceiling of `device`: pointer chase over a `ws_bytes` buffer of
  * `line_bytes` (64 or 128) lines in one pseudo-random cycle, waves_per_cu * CUs

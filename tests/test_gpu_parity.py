@@ -53,7 +53,7 @@ def readset_for(name, key, golden_dir):
         _index_cache[name] = nv.Index(words, meta, 0)
     o = nv.gap_opt_from_args(opts)
     bopts, nst, md = nv.prepare_batches(o, r["qlen"], r["batch"])
-    rs = nv.ReadSet(_index_cache[name], r["query"], r["qoff"], r["qlen"], md, r["batch"], bopts, nst, o.seed_len)
+    rs = nv.ReadSet(_index_cache[name], r["query"], r["qoff"], r["qlen"], md, r["batch"], bopts, nst)
     return w, o, rs
 
 
@@ -127,7 +127,7 @@ def test_reads_outside_limits_are_reported():
     batch = np.zeros(3, np.int32)
     bopts, nst, md = nv.prepare_batches(o, lens.astype(np.int32), batch)
     with nv.Index(words, meta, 0) as idx:
-        rs = nv.ReadSet(idx, q, qo, lens.astype(np.int32), md, batch, bopts, nst, o.seed_len)
+        rs = nv.ReadSet(idx, q, qo, lens.astype(np.int32), md, batch, bopts, nst)
         for form in (nv.FORM_PLAIN, nv.FORM_BATCHED):
             rs.search(form)
             res = rs.results()

@@ -33,10 +33,13 @@ def test_libraries_load_and_export_every_declared_function(header, lib):
     L = C.CDLL(lib)
     missing = [n for n in names if not hasattr(L, n)]
     assert not missing, missing
+    # and the ctypes mirror binds every one of them
+    mirrored = nv.HIP_SYMBOLS if header == "galn_hip.h" else nv.HOST_SYMBOLS
+    assert sorted(mirrored) == names
 
 
 def test_abi_version_and_limits():
-    assert nv.hip_lib().galn_hip_abi_version() == 1
+    assert nv.hip_lib().galn_hip_abi_version() == 2
     lim = nv.limits()
     assert lim["max_read_len"] == 1023 and lim["max_n_stacks"] == 256 and lim["max_seq_len"] == (1 << 38) - 1
 
