@@ -9,8 +9,9 @@ searched with `aln -l 1024 -n 0.01 -o 2`.
 
 A step is one search of the whole read set with the reads and the index resident in
 HBM, from the call into the library until every record is in host memory. The bench
-times K steps of the batched form (the reported value) and K steps of the plain form,
-checks that both forms give identical results, times the reference's CPU search
+times K steps of the batched form (the reported value) and --plain-steps steps of the
+plain form, checks that both forms give identical results, runs the serving loop under
+Poisson arrivals at 25/50/75/90% of its peak (p50/p99 request latency), times the reference's CPU search
 (oracle/_ref, built from the reference's C sources) on a bounded sample of the same
 reads and checks the GPU records against it, and reports the dominant kernel's
 achieved algorithmic bytes/s against MI355X's HBM peak and the measured
@@ -122,9 +123,10 @@ def serving_sweep(w, index, nv, loads, seconds: float, max_req: int, seed: int =
         ql = lens32[lo:hi]
         bid = (np.arange(size) // 0x40000).astype(np.int32)
         bopts, nst, md = nv.prepare_batches(o, ql, bid)
-        q0 = int(w.qoff[lo])
-        q1 = int(w.qoff[hi - 1]) + int(ql[-1])
-        return (w.query[q0:q1], w.qoff[lo:hi] - q0, ql, md, bid, bopts, nst)
+        qo = w.qoff[lo:hi]
+        q0 = int(qo.min())
+        q1 = int((qo + ql).max())
+        return (w.query[q0:q1], qo - np.uint64(q0), ql, md, bid, bopts, nst)
 
     def run(srv, reqs, rate):
         """Submit reqs (at `rate` requests/s, or back to back if None); latencies in ms."""
@@ -200,8 +202,12 @@ def main() -> int:
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--plain-steps", type=int, default=1, help="timed steps of the plain form (it is ~20x slower)")
     ap.add_argument("--serve-seconds", type=float, default=8.0, help="arrival time per offered load (0: no serving)")
     ap.add_argument("--serve-max-request", type=int, default=1_000_000)
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="also copy a buffer of this many MiB device to device three times (the known-size "
+                         "kernel that scales the FETCH_SIZE / WRITE_SIZE counters)")
     ap.add_argument("--workdir", default="")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -243,9 +249,6 @@ def main() -> int:
     for _ in range(max(args.warmup, 0)):
         rs.search(nv.FORM_BATCHED)
         rs.results(want_pops=False)
-    if not args.no_plain:
-        rs.search(nv.FORM_PLAIN)
-        res_plain = rs.results(want_pops=False)
 
     def timed(form: int, steps: int):
         times, launches = [], []
@@ -264,10 +267,9 @@ def main() -> int:
     log(f"timing batched form, {args.steps} steps")
     tb, lb, res_b = timed(nv.FORM_BATCHED, args.steps)
     if not args.no_plain:
-        log(f"timing plain form, {args.steps} steps")
-        tp, lp, res_p = timed(nv.FORM_PLAIN, args.steps)
-        same = (np.array_equal(res_p.n_aln, res_b.n_aln) and np.array_equal(res_p.rec, res_b.rec)
-                and np.array_equal(res_plain.n_aln, res_b.n_aln))
+        log(f"timing plain form, {args.plain_steps} steps")
+        tp, lp, res_p = timed(nv.FORM_PLAIN, args.plain_steps)
+        same = np.array_equal(res_p.n_aln, res_b.n_aln) and np.array_equal(res_p.rec, res_b.rec)
     else:
         tp, lp, same = [], [], None
     log(f"plain == batched: {same}")
@@ -298,7 +300,7 @@ def main() -> int:
                    "ref_bp": args.ref_bp, "reads": w.n, "aln_opts": " ".join(w.opts), "form": "batched",
                    "index_bytes": ws},
         "plain": {"value": w.n / (sum(tp) / len(tp)) if tp else None,
-                  "ms_per_step": 1e3 * sum(tp) / len(tp) if tp else None},
+                  "ms_per_step": 1e3 * sum(tp) / len(tp) if tp else None, "steps": len(tp)},
         "batched_over_plain": (sum(tp) / len(tp)) / (sum(tb) / len(tb)) if tp else None,
         "parity": {"plain_eq_batched": same},
         "kernel": {"name": "k_search<persistent>", "first_tier_ms": k_ms,
@@ -309,6 +311,15 @@ def main() -> int:
         "ceiling": {"kind": "pointer chase, 64-byte random lines", "ws_bytes": ws, "GB/s": ceiling_gbs,
                     "frac": achieved / ceiling_gbs if ceiling_gbs else None},
     }
+    if args.copy_calibration > 0:
+        n = args.copy_calibration << 20
+        src = torch.ones(n, dtype=torch.uint8, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        log(f"copy calibration: 3 device copies of {n} bytes")
+        del src, dst
     if args.serve_seconds > 0:
         out["serving"] = serving_sweep(w, index, nv, (0.25, 0.5, 0.75, 0.9), args.serve_seconds,
                                        min(args.serve_max_request, w.n))

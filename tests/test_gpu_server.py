@@ -35,9 +35,10 @@ def requests(golden_dir):
         key = ("default_1", "adna_1")[j % 2]
         r, bopts, nst, md = runs[key]
         hi = min(lo + size, r["n"])
-        q0 = int(r["qoff"][lo])
-        q1 = int(r["qoff"][hi - 1]) + int(r["qlen"][hi - 1])
-        arrays = (r["query"][q0:q1], r["qoff"][lo:hi] - q0, r["qlen"][lo:hi], md[lo:hi], r["batch"][lo:hi], bopts, nst)
+        qo, ql = r["qoff"][lo:hi], r["qlen"][lo:hi]
+        q0 = int(qo.min())
+        q1 = int((qo + ql).max())
+        arrays = (r["query"][q0:q1], qo - np.uint64(q0), ql, md[lo:hi], r["batch"][lo:hi], bopts, nst)
         out.append((key, lo, hi, arrays))
         lo = hi % r["n"]
     return w, out
