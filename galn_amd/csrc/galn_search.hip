@@ -63,6 +63,9 @@ constexpr int ERR_UNSUP = -5;
 constexpr int ERR_OUT = -8;     // output arena full: re-run after it grows
 constexpr int ERR_INTERNAL = -9; // stack bookkeeping inconsistent (final)
 constexpr int ERR_HEAVY = -10;   // pop budget spent: the read moves to the wave-per-read engine
+constexpr int NOFIN = 0x7fffffff;  // k_search helpers: the read goes on
+constexpr int HITP = 0x7ffffffe;   // k_search pop: the popped entry is a hit
+constexpr int STARTS = 0x7ffffffd; // k_search widths: done, the search is set up
 
 constexpr int MODE_GAPE = 0x01, MODE_LOGGAP = 0x04, MODE_NONSTOP = 0x10;
 constexpr int ST_M = 0, ST_I = 1, ST_D = 2;
@@ -225,7 +228,10 @@ __device__ __forceinline__ int wbid(uint64_t w) { return (int)(w >> 48); }
 __device__ __forceinline__ uint64_t wval(uint64_t w) { return w & W48; }
 __device__ __forceinline__ uint64_t wpack(uint64_t w, int bid) { return w | ((uint64_t)bid << 48); }
 
-enum : int { PH_CW = 0, PH_POPPED = 1, PH_EXACT = 2 };
+// what a lane of k_search does with the loads of an iteration: claim a read (no loads), a
+// width step, evaluate a popped entry, expand (or prune, or start the exact tail of) an entry,
+// one exact-tail step
+enum : int { PH_CLAIM = 0, PH_CW = 1, PH_POP = 2, PH_EXP = 3, PH_EXACT = 4 };
 
 // Free granules: NSHARD lock-free stacks, each with a head that carries a tag bumped on every
 // change (no ABA), links read and written at device scope; then never-used granules. A
@@ -302,11 +308,10 @@ __global__ void __launch_bounds__(128) k_search(KArgs a)
     int rid = 0, len = 0;
     const uint8_t *q = nullptr;
     int s_mm = 0, s_gapo = 0, s_gape = 0, mode = 0, ies = 0, mdo = 0, max_entries = 0, max_gapo = 0, max_gape = 0,
-        msd = 0, max_top2 = 0, n_stacks = 0, md_read = 0;
+        msd = 0, max_top2 = 0, n_stacks = 0, md_read = 0, seed_len = 0;
     bool seeding = false;
-    int seed_len = 0;
-    // chain (widths / exact tail)
-    int phase = PH_CW, cj = 0, plen = 0, cbid = 0, cbase = 0;
+    // widths in progress: position cj of `pstr` (the query, or its seed part), interval [ck, cl]
+    int cj = 0, plen = 0, cbid = 0, cbase = 0, nN = 0;
     const uint8_t *pstr = nullptr;
     uint64_t *wout = nullptr;
     bool in_seed = false;
@@ -314,18 +319,22 @@ __global__ void __launch_bounds__(128) k_search(KArgs a)
     // search
     int best_score = 0, best_diff = 0, max_diff = 0, best_cnt = 0, n_aln = 0, n_entries = 0;
     uint64_t bm0 = 0, bm1 = 0, bm2 = 0, bm3 = 0;
-    bool has_cur = false;
-    Ent cur{0, 0}, e{0, 0};
+    Ent cur{0, 0}, e{0, 0}, pf{0, 0};
+    bool has_cur = false, pf_valid = false;
     int cur_score = 0, e_score = 0, m = 0, m_seed = 0;
+    uint32_t pf_addr = NIL;
+    uint64_t hk = 0, hl = 0;  // a hit waiting to be applied
     uint32_t spare = NIL, free_head = NIL;
     int n_own = 0, own_next = 0, gleft = 0;  // granules held, next one to use, chunks left in it
     uint32_t gbase = 0;
     uint64_t pops = 0, trace = TR_INIT, obytes = 0;
-    uint32_t lf_bytes = 0;
-    // LF request
-    bool need_lf = false;
+    // what the next iteration loads, and what it then does
+    int phase = PH_CLAIM;
+    bool rq_lf = false, rq_ent = false, rq_pf = false, rq_w = false;
     uint64_t qa = 0, qb = 0;
-    bool need_claim = true, claimed = false;
+    uint32_t ent_addr = 0;
+    const uint8_t *c1p = nullptr, *c2p = nullptr;  // query bytes to load (null: none)
+    bool claimed = false;
 
     // ---- helpers (all lane-local) ----
     auto bm_test = [&](int s) __attribute__((always_inline)) -> bool {
@@ -348,6 +357,7 @@ __global__ void __launch_bounds__(128) k_search(KArgs a)
         bm2 &= w == 2 ? bit : ~0ull;
         bm3 &= w == 3 ? bit : ~0ull;
     };
+    auto bm_any = [&]() __attribute__((always_inline)) -> bool { return (bm0 | bm1 | bm2 | bm3) != 0; };
     auto bm_low = [&]() __attribute__((always_inline)) -> int {
         return bm0 ? __builtin_ctzll(bm0)
                    : bm1 ? 64 + __builtin_ctzll(bm1) : bm2 ? 128 + __builtin_ctzll(bm2) : 192 + __builtin_ctzll(bm3);
@@ -417,17 +427,6 @@ __global__ void __launch_bounds__(128) k_search(KArgs a)
         ++n_entries;
         return 0;
     };
-    // a child: the exact one (same score, pushed last) is kept in registers
-    auto push = [&](int s, const Ent &en, bool exact) __attribute__((always_inline)) -> int {
-        if (exact && a.fuse) {
-            cur = en;
-            cur_score = s;
-            has_cur = true;
-            ++n_entries;
-            return 0;
-        }
-        return push_mem(s, en);
-    };
     auto score_of = [&](int mm, int go, int ge) __attribute__((always_inline)) -> int { return mm * s_mm + go * s_gapo + ge * s_gape; };
 
     // result of the current read; the lane then claims the next one
@@ -435,63 +434,64 @@ __global__ void __launch_bounds__(128) k_search(KArgs a)
         int n = code;
         if (n == ERR_HEAVY) {
             a.heavy_list[atomicAdd(a.heavy_n, 1)] = rid;
-            if (n_own > a.keep) {
-                gran_push(a.ghead, a.gnext, OWN, a.keep, n_own);
-                n_own = a.keep;
+        } else {
+            if (n > 0) {
+                const unsigned long long b0 = atomicAdd(a.out_top, (unsigned long long)n);
+                if (b0 + (unsigned long long)n > a.out_cap) {
+                    n = ERR_OUT;
+                } else {
+                    for (int j = 0; j < n; ++j) a.out_rec[b0 + j] = REC[j];
+                    a.out_off[rid] = b0;
+                }
             }
-            need_lf = false;
-            need_claim = true;
-            return;
-        }
-        if (n > 0) {
-            const unsigned long long b0 = atomicAdd(a.out_top, (unsigned long long)n);
-            if (b0 + (unsigned long long)n > a.out_cap) {
-                n = ERR_OUT;
-            } else {
-                for (int j = 0; j < n; ++j) a.out_rec[b0 + j] = REC[j];
-                a.out_off[rid] = b0;
+            if (n == ERR_STACK || n == ERR_ALN || n == ERR_OUT) {
+                const int f = atomicAdd(a.fail_n, 1);
+                a.fail_list[f] = rid;
+                atomicOr(a.fail_kind + (n == ERR_OUT ? 0 : n == ERR_ALN ? 1 : 2), 1);
             }
+            a.out_n[rid] = n;
+            a.out_pops[rid] = pops;
+            if (TRACE) a.out_trace[rid] = trace;
+            if (a.out_bytes) a.out_bytes[rid] = obytes;
+            if (n != ERR_STACK && n != ERR_ALN && n != ERR_OUT) atomicAdd(a.algo_bytes, (unsigned long long)obytes);
         }
-        if (n == ERR_STACK || n == ERR_ALN || n == ERR_OUT) {
-            const int f = atomicAdd(a.fail_n, 1);
-            a.fail_list[f] = rid;
-            atomicOr(a.fail_kind + (n == ERR_OUT ? 0 : n == ERR_ALN ? 1 : 2), 1);
-        }
-        a.out_n[rid] = n;
-        a.out_pops[rid] = pops;
-        if (TRACE) a.out_trace[rid] = trace;
-        if (a.out_bytes) a.out_bytes[rid] = obytes;
-        if (n != ERR_STACK && n != ERR_ALN && n != ERR_OUT) atomicAdd(a.algo_bytes, (unsigned long long)obytes);
         if (n_own > a.keep) {
             gran_push(a.ghead, a.gnext, OWN, a.keep, n_own);
             n_own = a.keep;
         }
-        need_lf = false;
-        need_claim = true;
+        rq_lf = rq_ent = rq_pf = rq_w = false;
+        c1p = c2p = nullptr;
+        phase = PH_CLAIM;
     };
 
-    // gap_shadow: src/bwtgap_hd.h:309-320
+    // gap_shadow: src/bwtgap_hd.h:309-320 (widths read eight at a time)
     auto shadow = [&](int x, int ldp) __attribute__((always_inline)) {
         const uint64_t ux = (uint64_t)(int64_t)x;
         int j = 0;
-        for (int i = 0; i < ldp; ++i) {
-            const uint64_t wv = W[i];
-            uint64_t w = wval(wv);
-            int bid = wbid(wv);
-            if (w > ux) {
-                w -= ux;
-            } else if (w == ux) {
-                bid = 1;
-                w = seq_len - (uint64_t)(++j);
-            } else {
-                continue;
+        for (int i0 = 0; i0 < ldp; i0 += 8) {
+            uint64_t wv[8];
+#pragma unroll
+            for (int t = 0; t < 8; ++t) wv[t] = i0 + t < ldp ? W[i0 + t] : 0;
+#pragma unroll
+            for (int t = 0; t < 8; ++t) {
+                if (i0 + t >= ldp) break;
+                uint64_t w = wval(wv[t]);
+                int bid = wbid(wv[t]);
+                if (w > ux) {
+                    w -= ux;
+                } else if (w == ux) {
+                    bid = 1;
+                    w = seq_len - (uint64_t)(++j);
+                } else {
+                    continue;
+                }
+                W[i0 + t] = wpack(w, bid);
             }
-            W[i] = wpack(w, bid);
         }
     };
 
-    // a hit (src/bwtgap_hd.h:385-409); returns true when the search stops here
-    auto hit = [&](uint64_t k, uint64_t l) __attribute__((always_inline)) -> bool {
+    // a hit (src/bwtgap_hd.h:385-409): NOFIN, or the code the read finishes with
+    auto hit = [&](uint64_t k, uint64_t l) __attribute__((always_inline)) -> int {
         const int mm = e_mm(e), go = e_go(e), ge = e_ge(e);
         const int score = e_score;
         bool do_add = true;
@@ -504,8 +504,7 @@ __global__ void __launch_bounds__(128) k_search(KArgs a)
         if (score == best_score) {
             best_cnt = (int)(uint32_t)((uint64_t)(int64_t)best_cnt + (l - k + 1));
         } else if (best_cnt > max_top2) {
-            finish(n_aln);
-            return true;
+            return n_aln;
         }
         if (go) {
             for (int j = 0; j < n_aln; ++j)
@@ -516,10 +515,7 @@ __global__ void __launch_bounds__(128) k_search(KArgs a)
         }
         if (do_add) {
             shadow((int)(uint32_t)(l - k + 1), e_ldp(e));
-            if (n_aln == a.rec_cap) {
-                finish(ERR_ALN);
-                return true;
-            }
+            if (n_aln == a.rec_cap) return ERR_ALN;
             const int ins = e_ins(e), del = go + ge - ins;
             galn_aln_rec_t r;
             r.packed = ((uint64_t)(mm & 0xff)) | ((uint64_t)(go & 0xff) << 8) | ((uint64_t)(ge & 0xff) << 16) |
@@ -530,49 +526,62 @@ __global__ void __launch_bounds__(128) k_search(KArgs a)
             REC[n_aln] = r;
             ++n_aln;
         }
-        return false;
+        return NOFIN;
     };
 
-    // the top of bwt_match_gap's loop: pop entries until one needs an LF lookup
-    // or the search ends (src/bwtgap_hd.h:357-383)
-    auto advance = [&]() __attribute__((always_inline)) {
+    // The top of bwt_match_gap's loop (src/bwtgap_hd.h:357-383): pop entries until one needs
+    // an LF lookup (its loads are requested for the next iteration: NOFIN), an entry has to be
+    // loaded first (NOFIN), a popped entry is a hit (HITP, at (hk, hl)), or the search ends
+    // (the finish code). have_e: `e` was just loaded and not yet evaluated.
+    auto pop_loop = [&](bool have_e) __attribute__((always_inline)) -> int {
         for (;;) {
-            if (n_entries == 0 || n_entries > max_entries) {
-                finish(n_aln);
-                return;
-            }
-            if (has_cur) {
-                e = cur;
-                e_score = cur_score;
-                has_cur = false;
-            } else {
-                if (!(bm0 | bm1 | bm2 | bm3)) {  // entries counted but none held: never expected
-                    finish(ERR_INTERNAL);
-                    return;
-                }
-                const int s = bm_low();
-                TopT *tp = tops + s * tstride;
-                const uint32_t t = *tp;
-                const uint32_t chunk = t >> 4, idx = t & 15u;
-                e = CHK[(uint64_t)chunk * CH + idx];
-                e_score = s;
-                if (idx > 0) {
-                    *tp = (TopT)(t - 1);
+            if (!have_e) {
+                if (n_entries == 0 || n_entries > max_entries) return n_aln;
+                if (has_cur) {
+                    e = cur;
+                    e_score = cur_score;
+                    has_cur = false;
                 } else {
-                    const uint32_t p = LINK[chunk];
-                    chunk_free(chunk);
-                    if (p != NIL)
-                        *tp = (TopT)(p << 4 | (CH - 1));
-                    else
-                        bm_clear(s);
+                    if (!bm_any()) return ERR_INTERNAL;  // entries counted but none held: never expected
+                    const int s = bm_low();
+                    TopT *tp = tops + s * tstride;
+                    const uint32_t t = *tp;
+                    const uint32_t chunk = t >> 4, idx = t & 15u;
+                    if (idx > 0) {
+                        *tp = (TopT)(t - 1);
+                    } else {
+                        const uint32_t p = LINK[chunk];
+                        chunk_free(chunk);  // its entries stay readable until a later push
+                        if (p != NIL)
+                            *tp = (TopT)(p << 4 | (CH - 1));
+                        else
+                            bm_clear(s);
+                    }
+                    e_score = s;
+                    if (pf_valid && pf_addr == t) {
+                        e = pf;
+                        pf_valid = false;
+                    } else {
+                        ent_addr = t;
+                        rq_ent = true;
+                        phase = PH_POP;
+                        // and the entry now on top of the lowest bucket
+                        if (bm_any()) {
+                            const uint32_t t2 = tops[bm_low() * tstride];
+                            if (!pf_valid || pf_addr != t2) {
+                                pf_addr = t2;
+                                pf_valid = false;
+                                rq_pf = true;
+                            }
+                        }
+                        return NOFIN;
+                    }
                 }
             }
+            have_e = false;
             --n_entries;
             ++pops;
-            if (a.pop_budget && pops > a.pop_budget) {
-                finish(ERR_HEAVY);
-                return;
-            }
+            if (a.pop_budget && pops > a.pop_budget) return ERR_HEAVY;
             const int i0 = e_i(e), mm = e_mm(e), go = e_go(e), ge = e_ge(e), st = e_st(e);
             if (TRACE) {
                 const int ins = e_ins(e), del = go + ge - ins;
@@ -584,10 +593,7 @@ __global__ void __launch_bounds__(128) k_search(KArgs a)
                 trace = tr_mix(trace, (uint64_t)(uint32_t)e_ldp(e));
                 trace = tr_mix(trace, (uint64_t)(uint32_t)n_entries);
             }
-            if (!(mode & MODE_NONSTOP) && (uint32_t)e_score > (uint32_t)(best_score + s_mm)) {
-                finish(n_aln);
-                return;
-            }
+            if (!(mode & MODE_NONSTOP) && (uint32_t)e_score > (uint32_t)(best_score + s_mm)) return n_aln;
             m = max_diff - (mm + go);
             if (mode & MODE_GAPE) m -= ge;
             if (m < 0) continue;
@@ -596,22 +602,33 @@ __global__ void __launch_bounds__(128) k_search(KArgs a)
                 if (mode & MODE_GAPE) m_seed -= ge;
             }
             if (i0 == 0) {
-                if (hit(e_k(e), e_l(e))) return;
-                continue;
+                hk = e_k(e);
+                hl = e_l(e);
+                return HITP;
             }
-            // the next LF lookup is at (k-1, l) whether the entry expands or
-            // starts an exact tail: issue it with the width loads
+            // the LF lookup at (k-1, l), whether the entry expands or starts an exact tail,
+            // with the widths and the query bytes it needs
             qa = e_k(e) - 1;
             qb = e_l(e);
-            need_lf = true;
-            phase = PH_POPPED;
-            return;
+            rq_lf = true;
+            rq_w = true;
+            c1p = q + (i0 - 1);
+            c2p = i0 >= 2 ? q + (i0 - 2) : nullptr;
+            phase = PH_EXP;
+            // and the entry on top of the lowest bucket, usually the next pop
+            if (bm_any()) {
+                const uint32_t t = tops[bm_low() * tstride];
+                if (!pf_valid || pf_addr != t) {
+                    pf_addr = t;
+                    pf_valid = false;
+                    rq_pf = true;
+                }
+            }
+            return NOFIN;
         }
     };
 
-    auto start_search = [&]() __attribute__((always_inline)) {
-        int nN = 0;
-        for (int j = 0; j < len; ++j) nN += q[j] > 3;
+    auto start_search = [&]() __attribute__((always_inline)) -> int {
         best_score = score_of(md_read + 1, max_gapo + 1, max_gape + 1);
         best_diff = md_read + 1;
         max_diff = md_read;
@@ -625,21 +642,26 @@ __global__ void __launch_bounds__(128) k_search(KArgs a)
         own_next = 0;
         gleft = 0;
         has_cur = false;
-        if (nN > md_read) {
-            finish(0);
-            return;
-        }
+        pf_valid = false;
+        if (nN > md_read) return 0;
+        if (n_stacks <= 0) return ERR_SCORE;
         // root: (i = len, k = 0, l = seq_len), score 0
-        const int r = push(0, mk_ent(0, seq_len, len, 0, 0, 0, 0, 0, ST_M), n_stacks > 0);
-        if (r) {
-            finish(r);
-            return;
+        const Ent root = mk_ent(0, seq_len, len, 0, 0, 0, 0, 0, ST_M);
+        if (a.fuse) {
+            cur = root;
+            cur_score = 0;
+            has_cur = true;
+            ++n_entries;
+            return NOFIN;
         }
-        advance();
+        const int r = push_mem(0, root);
+        return r ? r : NOFIN;
     };
 
-    // widths: step to the next position that needs an LF lookup (N restarts inline)
-    auto prep_width = [&]() __attribute__((always_inline)) {
+    // widths (bwt_cal_width, src/bwtgap_hd.h:216-242): consume the query byte `c0` of position
+    // cj, loaded by the previous iteration; request the next LF lookup or byte (NOFIN), or,
+    // when the widths are done, set the search up (STARTS: pop next; or a finish code)
+    auto width_char = [&](int c0) __attribute__((always_inline)) -> int {
         for (;;) {
             if (cj == plen) {
                 wout[plen] = wpack(0, ++cbid);
@@ -652,30 +674,37 @@ __global__ void __launch_bounds__(128) k_search(KArgs a)
                     ck = 0;
                     cl = seq_len;
                     cbid = 0;
-                    continue;
+                    if (plen == 0) continue;
+                    c1p = pstr;
+                    return NOFIN;
                 }
-                start_search();
-                return;
+                const int r = start_search();
+                return r == NOFIN ? STARTS : r;
             }
-            const int c0 = pstr[cj];
-            const int c = c0 > 3 ? 4 : 3 - c0;
-            if (c < 4) {
-                cbase = c;
+            if (!in_seed) nN += c0 > 3;
+            if (c0 <= 3) {
+                cbase = 3 - c0;
                 qa = ck - 1;
                 qb = cl;
-                need_lf = true;
-                phase = PH_CW;
-                return;
+                rq_lf = true;
+                c1p = cj + 1 < plen ? pstr + (cj + 1) : nullptr;
+                return NOFIN;
             }
+            // N: restart without a lookup
             ck = 0;
             cl = seq_len;
             ++cbid;
             wout[cj] = wpack(seq_len + 1, cbid);
             ++cj;
+            if (cj < plen) {
+                c1p = pstr + cj;
+                return NOFIN;
+            }
+            c0 = 4;
         }
     };
 
-    auto init_read = [&](int pos) __attribute__((always_inline)) {
+    auto init_read = [&](int pos) __attribute__((always_inline)) -> int {
         rid = a.list ? a.list[pos] : a.list_base + pos;
         len = a.qlen[rid];
         q = a.query + a.qoff[rid];
@@ -699,11 +728,10 @@ __global__ void __launch_bounds__(128) k_search(KArgs a)
         pops = 0;
         trace = TR_INIT;
         obytes = 0;
+        pf_valid = false;
         if (len < 1 || len > MAXLEN || n_stacks > MAXB || md_read > MAXDIFF || max_gapo > MAXDIFF ||
-            max_gapo + max_gape > 255 || max_gape < 0) {
-            finish(ERR_UNSUP);
-            return;
-        }
+            max_gapo + max_gape > 255 || max_gape < 0)
+            return ERR_UNSUP;
         in_seed = false;
         pstr = q;
         plen = len;
@@ -712,12 +740,16 @@ __global__ void __launch_bounds__(128) k_search(KArgs a)
         ck = 0;
         cl = seq_len;
         cbid = 0;
-        prep_width();
+        nN = 0;
+        c1p = q;
+        phase = PH_CW;
+        return NOFIN;
     };
 
-    // ---- main loop: one LF lookup per lane per iteration ----
+    // ---- main loop: every iteration loads what the lane asked for, then works on it ----
     for (;;) {
-        if (need_claim) {
+        int fin = NOFIN;
+        if (phase == PH_CLAIM) {
             int pos;
             if (PERSIST) {
                 pos = atomicAdd(a.qnext, 1);
@@ -727,173 +759,227 @@ __global__ void __launch_bounds__(128) k_search(KArgs a)
             }
             claimed = true;
             if (pos >= a.n_list) break;
-            need_claim = false;
-            init_read(pos);
-            continue;
-        }
-
-        // the popped entry's widths and query base are loaded with its LF lookup
-        uint64_t wA = 0, wB = 0, sA = 0, sB = 0;
-        int qbase = 0;
-        if (phase == PH_POPPED) {
-            const int i0 = e_i(e);
-            wB = W[i0 - 1];
-            if (i0 >= 2) wA = W[i0 - 2];
-            qbase = q[i0 - 1];
-            if (seeding) {
-                const int ii = i0 - 1 - (len - seed_len);
-                if (ii > 0) {
-                    sA = SW[ii - 1];
-                    sB = SW[ii];
-                }
-            }
-        }
-        uint64_t ca[4], cb[4];
-        if (need_lf) lf_bytes = lf_pair(a.bwt, M, qa, qb, ca, cb);
-        need_lf = false;
-
-        if (phase == PH_CW) {
-            obytes += lf_bytes;
-            const int c = cbase;
-            uint64_t k = M.L2[c] + ca[c] + 1, l = M.L2[c] + cb[c];
-            if (k > l) {
-                k = 0;
-                l = seq_len;
-                ++cbid;
-            }
-            wout[cj] = wpack(l - k + 1, cbid);
-            ck = k;
-            cl = l;
-            ++cj;
-            prep_width();
+            fin = init_read(pos);
         } else {
-            // exact tails (bwt_match_exact_alt, src/bwtgap_hd.h:201-208) and popped entries
-            // (src/bwtgap_hd.h:375-470) end in one of: the next LF lookup, a hit, or the next pop
-            int act = 0;  // 0: LF lookup set up, 1: pop next, 2: hit at (hk, hl) then pop next
-            uint64_t hk = 0, hl = 0;
-            bool expand = false;
-            int c = cbase, j_next = cj - 1;
-            if (phase == PH_POPPED) {
-                const int i0 = e_i(e);
-                const int st = e_st(e), ge = e_ge(e);
-                if (m < wbid(wB)) {
-                    act = 1;
-                } else if (m == 0 && (st == ST_M || (mode & MODE_GAPE) || ge == max_gape)) {
-                    // exact tail: its first step is this lookup
-                    c = qbase;
-                    j_next = i0 - 2;
-                    phase = PH_EXACT;
-                    if (qbase > 3) act = 1;
-                } else {
-                    expand = true;
+            // ---- loads, all issued before any is used ----
+            uint64_t wA = 0, wB = 0, sA = 0, sB = 0;
+            int ch1 = 4, ch2 = 4;
+            Ent ld{0, 0};
+            uint64_t ca[4] = {0, 0, 0, 0}, cb[4] = {0, 0, 0, 0};
+            uint32_t lf_bytes = 0;
+            if (rq_w) {
+                const int wi = e_i(e);
+                wB = W[wi - 1];
+                if (wi >= 2) wA = W[wi - 2];
+                if (seeding) {
+                    const int ii = wi - 1 - (len - seed_len);
+                    if (ii > 0) {
+                        sA = SW[ii - 1];
+                        sB = SW[ii];
+                    }
                 }
             }
-            if (phase == PH_EXACT && act == 0) {
+            if (c1p) ch1 = *c1p;
+            if (c2p) ch2 = *c2p;
+            if (rq_ent) ld = CHK[(uint64_t)(ent_addr >> 4) * CH + (ent_addr & 15u)];
+            if (rq_pf) pf = CHK[(uint64_t)(pf_addr >> 4) * CH + (pf_addr & 15u)];
+            if (rq_lf) lf_bytes = lf_pair(a.bwt, M, qa, qb, ca, cb);
+            if (rq_pf) pf_valid = true;
+            const bool had_lf = rq_lf;
+            rq_lf = rq_ent = rq_pf = rq_w = false;
+            c1p = c2p = nullptr;
+
+            // ---- work: each phase ends in new loads, a hit and/or a pop, or a finish ----
+            bool want_pop = false, have_e = false, want_hit = false;
+            if (phase == PH_CW) {
+                if (had_lf) {
+                    obytes += lf_bytes;
+                    const int c = cbase;
+                    uint64_t k = M.L2[c] + ca[c] + 1, l = M.L2[c] + cb[c];
+                    if (k > l) {
+                        k = 0;
+                        l = seq_len;
+                        ++cbid;
+                    }
+                    wout[cj] = wpack(l - k + 1, cbid);
+                    ck = k;
+                    cl = l;
+                    ++cj;
+                }
+                const int r = width_char(ch1);
+                if (r == STARTS)
+                    want_pop = true;
+                else
+                    fin = r;
+            } else if (phase == PH_POP) {
+                e = ld;
+                want_pop = have_e = true;
+            } else if (phase == PH_EXACT) {
+                // one step of an exact tail (bwt_match_exact_alt, src/bwtgap_hd.h:201-208)
                 obytes += lf_bytes;
+                const int c = cbase;
                 const uint64_t k = M.L2[c] + ca[c] + 1, l = M.L2[c] + cb[c];
-                if (k > l) {
-                    act = 1;
-                } else if (j_next < 0) {
-                    act = (int)(uint32_t)(l - k + 1) != 0 ? 2 : 1;
-                    hk = k;
-                    hl = l;
-                } else {
-                    const int c2 = q[j_next];
-                    if (c2 > 3) {
-                        act = 1;
-                    } else {
-                        cj = j_next;
-                        cbase = c2;
-                        ck = k;
-                        cl = l;
+                want_pop = true;
+                if (k <= l) {
+                    if (cj == 0) {
+                        want_hit = (int)(uint32_t)(l - k + 1) != 0;
+                        hk = k;
+                        hl = l;
+                    } else if (ch1 <= 3) {
+                        --cj;
+                        cbase = ch1;
                         qa = k - 1;
                         qb = l;
-                        need_lf = true;
+                        rq_lf = true;
+                        c1p = cj >= 1 ? q + (cj - 1) : nullptr;
+                        want_pop = false;
                     }
                 }
-            }
-            if (act == 2) {
-                if (hit(hk, hl)) continue;
-                act = 1;
-            }
-            if (act == 1) {
-                advance();
-                continue;
-            }
-            if (!expand) continue;
-            const int i0 = e_i(e);
-            const int st = e_st(e), ge = e_ge(e);
-            // expansion
-            obytes += lf_bytes;
-            const int i = i0 - 1;
-            const int mm = e_mm(e), go = e_go(e), ins = e_ins(e), del = go + ge - ins;
-            const uint64_t k = e_k(e), l = e_l(e), occ = l - k + 1;
-            bool allow_diff = true, allow_M = true;
-            if (i > 0) {
-                if (wbid(wA) > m - 1)
-                    allow_diff = false;
-                else if (wbid(wA) == m - 1 && wbid(wB) == m - 1 && wval(wA) == wval(wB))
-                    allow_M = false;
-                if (seeding) {
-                    const int ii = i - (len - seed_len);
-                    if (ii > 0) {
-                        if (wbid(sA) > m_seed - 1)
-                            allow_diff = false;
-                        else if (wbid(sA) == m_seed - 1 && wbid(sB) == m_seed - 1 && wval(sA) == wval(sB))
-                            allow_M = false;
-                    }
-                }
-            }
-            const int tmp = (mode & MODE_LOGGAP) ? int_log2((uint32_t)(ge + go)) / 2 + 1 : go + ge;
-            int err = 0;
-            if (allow_diff && i >= ies + tmp && len - i >= ies + tmp) {
-                if (st == ST_M) {
-                    if (go < max_gapo) {
-                        err = push(score_of(mm, go + 1, ge), mk_ent(k, l, i, i, mm, go + 1, ge, ins + 1, ST_I), false);
-                        for (int j = 0; j < 4 && !err; ++j) {
-                            const uint64_t kk = M.L2[j] + ca[j] + 1, ll = M.L2[j] + cb[j];
-                            if (kk <= ll)
-                                err = push(score_of(mm, go + 1, ge), mk_ent(kk, ll, i + 1, i + 1, mm, go + 1, ge, ins, ST_D),
-                                           false);
+            } else {
+                // PH_EXP: the popped entry's lookup, widths and query bytes are here
+                const int i0 = e_i(e);
+                const int st = e_st(e), ge = e_ge(e);
+                const int qbase = ch1;
+                want_pop = true;
+                if (m < wbid(wB)) {
+                    // pruned by the widths
+                } else if (m == 0 && (st == ST_M || (mode & MODE_GAPE) || ge == max_gape)) {
+                    // exact tail: its first step is this lookup
+                    if (qbase <= 3) {
+                        obytes += lf_bytes;
+                        const uint64_t k = M.L2[qbase] + ca[qbase] + 1, l = M.L2[qbase] + cb[qbase];
+                        if (k <= l) {
+                            if (i0 < 2) {
+                                want_hit = (int)(uint32_t)(l - k + 1) != 0;
+                                hk = k;
+                                hl = l;
+                            } else if (ch2 <= 3) {
+                                cj = i0 - 2;
+                                cbase = ch2;
+                                qa = k - 1;
+                                qb = l;
+                                rq_lf = true;
+                                c1p = cj >= 1 ? q + (cj - 1) : nullptr;
+                                phase = PH_EXACT;
+                                want_pop = false;
+                            }
                         }
                     }
-                } else if (st == ST_I) {
-                    if (ge < max_gape)
-                        err = push(score_of(mm, go, ge + 1), mk_ent(k, l, i, i, mm, go, ge + 1, ins + 1, ST_I), false);
                 } else {
-                    if (ge < max_gape && (ge + go < max_diff || occ < (uint64_t)(int64_t)mdo)) {
-                        for (int j = 0; j < 4 && !err; ++j) {
-                            const uint64_t kk = M.L2[j] + ca[j] + 1, ll = M.L2[j] + cb[j];
-                            if (kk <= ll)
-                                err = push(score_of(mm, go, ge + 1), mk_ent(kk, ll, i + 1, i + 1, mm, go, ge + 1, ins, ST_D),
-                                           false);
+                    // expansion: src/bwtgap_hd.h:412-470
+                    obytes += lf_bytes;
+                    const int i = i0 - 1;
+                    const int mm = e_mm(e), go = e_go(e), ins = e_ins(e);
+                    const uint64_t k = e_k(e), l = e_l(e), occ = l - k + 1;
+                    bool allow_diff = true, allow_M = true;
+                    if (i > 0) {
+                        if (wbid(wA) > m - 1)
+                            allow_diff = false;
+                        else if (wbid(wA) == m - 1 && wbid(wB) == m - 1 && wval(wA) == wval(wB))
+                            allow_M = false;
+                        if (seeding) {
+                            const int ii = i - (len - seed_len);
+                            if (ii > 0) {
+                                if (wbid(sA) > m_seed - 1)
+                                    allow_diff = false;
+                                else if (wbid(sA) == m_seed - 1 && wbid(sB) == m_seed - 1 && wval(sA) == wval(sB))
+                                    allow_M = false;
+                            }
                         }
                     }
-                }
-            }
-            (void)del;
-            if (!err) {
-                if (allow_diff && allow_M) {
-                    for (int j = 1; j <= 4 && !err; ++j) {
-                        const int c = (qbase + j) & 3;
-                        const int is_mm = (j != 4 || qbase > 3);
-                        const uint64_t kk = M.L2[c] + ca[c] + 1, ll = M.L2[c] + cb[c];
-                        if (kk <= ll)
-                            err = push(score_of(mm + is_mm, go, ge), mk_ent(kk, ll, i, is_mm ? i : 0, mm + is_mm, go, ge, ins, ST_M),
-                                       !is_mm);
+                    const int tmp = (mode & MODE_LOGGAP) ? int_log2((uint32_t)(ge + go)) / 2 + 1 : go + ge;
+                    const bool gap_ok = allow_diff && i >= ies + tmp && len - i >= ies + tmp;
+                    // the children, in the order bwt_match_gap pushes them:
+                    // bit 0 insertion, bits 1-4 deletions (bases 0..3), bits 5-8 the bases
+                    // qbase+1 .. qbase+4 (the last one is the exact child), bit 9 the exact child
+                    // alone (when differences are not allowed here)
+                    uint32_t mask = 0;
+                    if (gap_ok) {
+                        const bool open = st == ST_M && go < max_gapo;
+                        if (open || (st == ST_I && ge < max_gape)) mask |= 1u;
+                        if (open || (st == ST_D && ge < max_gape && (ge + go < max_diff || occ < (uint64_t)(int64_t)mdo))) {
+    #pragma unroll
+                            for (int j = 0; j < 4; ++j)
+                                if (M.L2[j] + ca[j] + 1 <= M.L2[j] + cb[j]) mask |= 2u << j;
+                        }
                     }
-                } else if (qbase < 4) {
-                    const int c = qbase;
-                    const uint64_t kk = M.L2[c] + ca[c] + 1, ll = M.L2[c] + cb[c];
-                    if (kk <= ll) err = push(e_score, mk_ent(kk, ll, i, 0, mm, go, ge, ins, ST_M), true);
+                    if (allow_diff && allow_M) {
+    #pragma unroll
+                        for (int j = 1; j <= 4; ++j) {
+                            const int c = (qbase + j) & 3;
+                            if (ca[c] + 1 <= cb[c]) mask |= 16u << j;
+                        }
+                    } else if (qbase < 4 && ca[qbase] + 1 <= cb[qbase]) {
+                        mask |= 512u;
+                    }
+                    int err = 0;
+                    while (mask && !err) {
+                        const int bit = __builtin_ctz(mask);
+                        mask &= mask - 1;
+                        Ent ch;
+                        int sc;
+                        bool exact = false;
+                        if (bit == 0) {
+                            if (st == ST_M) {
+                                sc = score_of(mm, go + 1, ge);
+                                ch = mk_ent(k, l, i, i, mm, go + 1, ge, ins + 1, ST_I);
+                            } else {
+                                sc = score_of(mm, go, ge + 1);
+                                ch = mk_ent(k, l, i, i, mm, go, ge + 1, ins + 1, ST_I);
+                            }
+                        } else if (bit <= 4) {
+                            const int j = bit - 1;
+                            const uint64_t kk = M.L2[j] + ca[j] + 1, ll = M.L2[j] + cb[j];
+                            const int go2 = st == ST_M ? go + 1 : go, ge2 = st == ST_M ? ge : ge + 1;
+                            sc = score_of(mm, go2, ge2);
+                            ch = mk_ent(kk, ll, i + 1, i + 1, mm, go2, ge2, ins, ST_D);
+                        } else {
+                            const int j = bit - 4;  // 1..4, or 5 for the lone exact child
+                            const int c = bit == 9 ? qbase : (qbase + j) & 3;
+                            const int is_mm = bit != 9 && (j != 4 || qbase > 3);
+                            const uint64_t kk = M.L2[c] + ca[c] + 1, ll = M.L2[c] + cb[c];
+                            sc = score_of(mm + is_mm, go, ge);
+                            ch = mk_ent(kk, ll, i, is_mm ? i : 0, mm + is_mm, go, ge, ins, ST_M);
+                            exact = !is_mm;
+                        }
+                        if (exact && a.fuse) {
+                            // pushed last with the parent's score: the next pop; kept in registers
+                            cur = ch;
+                            cur_score = sc;
+                            has_cur = true;
+                            ++n_entries;
+                        } else {
+                            err = push_mem(sc, ch);
+                        }
+                    }
+                    if (err) {
+                        fin = err;
+                        want_pop = false;
+                    }
                 }
             }
-            if (err) {
-                finish(err);
-                continue;
+            // ---- hits and pops, until new loads are requested or the read finishes ----
+            for (;;) {
+                if (want_hit) {
+                    want_hit = false;
+                    const int h = hit(hk, hl);
+                    if (h != NOFIN) {
+                        fin = h;
+                        break;
+                    }
+                }
+                if (!want_pop) break;
+                const int r = pop_loop(have_e);
+                have_e = false;
+                if (r == HITP) {
+                    want_hit = true;
+                    continue;
+                }
+                fin = r;
+                break;
             }
-            advance();
         }
+        if (fin != NOFIN) finish(fin);
     }
 }
 
