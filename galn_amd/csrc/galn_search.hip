@@ -134,6 +134,16 @@ struct KArgs {
     int32_t *heavy_n;
 };
 
+// The kernel argument segment as k_search reads it, and a load that the compiler must perform
+// where it is written (not hoisted out of the loop into registers).
+typedef const __attribute__((address_space(4))) KArgs KArgsC;
+template <class T>
+__device__ __forceinline__ T cold_ld(const __attribute__((address_space(4))) T *p)
+{
+    return *reinterpret_cast<const volatile __attribute__((address_space(4))) T *>(p);
+}
+#define CA(f) cold_ld(&A->f)
+
 // ---------------------------------------------------------------------------
 // Occurrence counts. Index layout: 128-symbol blocks of 16 u32 words, 4 x u64
 // exclusive counts then 8 words of 16 two-bit symbols (first symbol in bits
@@ -291,6 +301,9 @@ __global__ void __launch_bounds__(128) k_search(KArgs a)
     extern __shared__ __align__(16) uint8_t lds_raw[];
     const int slot = blockIdx.x * blockDim.x + threadIdx.x;
     if (slot >= a.n_slots) return;
+    // arguments used once per read (claims, results, granules) are read from the kernel
+    // argument segment where they are used, so they do not hold scalar registers for the loop
+    const KArgsC *const A = (const KArgsC *)__builtin_amdgcn_kernarg_segment_ptr();
     TopT *const tops = reinterpret_cast<TopT *>(lds_raw) + threadIdx.x;
     const int tstride = blockDim.x;
 
@@ -322,7 +335,7 @@ __global__ void __launch_bounds__(128) k_search(KArgs a)
     Ent cur{0, 0}, e{0, 0}, pf{0, 0};
     bool has_cur = false, pf_valid = false;
     int cur_score = 0, e_score = 0, m = 0, m_seed = 0;
-    uint32_t pf_addr = NIL;
+    uint32_t pf_addr = NIL, ld_ntop = NIL;  // prefetched entry; lowest top after a pop being loaded
     uint64_t hk = 0, hl = 0;  // a hit waiting to be applied
     uint32_t spare = NIL, free_head = NIL;
     int n_own = 0, own_next = 0, gleft = 0;  // granules held, next one to use, chunks left in it
@@ -378,8 +391,8 @@ __global__ void __launch_bounds__(128) k_search(KArgs a)
             if (own_next < n_own) {
                 g = OWN[own_next];
             } else {
-                if (n_own >= a.own_cap) return NIL;
-                g = gran_pop(a.ghead, a.gnext, a.gbump, a.n_gran);
+                if (n_own >= CA(own_cap)) return NIL;
+                g = gran_pop(CA(ghead), CA(gnext), CA(gbump), CA(n_gran));
                 if (g == NIL) return NIL;
                 OWN[n_own++] = g;
             }
@@ -428,36 +441,68 @@ __global__ void __launch_bounds__(128) k_search(KArgs a)
         return 0;
     };
     auto score_of = [&](int mm, int go, int ge) __attribute__((always_inline)) -> int { return mm * s_mm + go * s_gapo + ge * s_gape; };
+    // n (<= 16) consecutive pushes into bucket s: the first n1 go to chunk c0 from slot s0,
+    // the rest to chunk c1 from slot 0; the bucket's top moves to the last
+    auto reserve = [&](int s, int n, uint32_t &c0, uint32_t &s0, uint32_t &n1, uint32_t &c1)
+        __attribute__((always_inline)) -> int {
+        if (s < 0 || s >= n_stacks) return ERR_SCORE;
+        TopT *tp = tops + s * tstride;
+        if (bm_test(s)) {
+            const uint32_t t = *tp;
+            c0 = t >> 4;
+            s0 = (t & 15u) + 1;
+        } else {
+            const uint32_t nc = chunk_alloc();
+            if (nc == NIL) return ERR_STACK;
+            LINK[nc] = NIL;
+            c0 = nc;
+            s0 = 0;
+            bm_set(s);
+        }
+        n1 = min((uint32_t)n, (uint32_t)CH - s0);
+        if ((uint32_t)n > n1) {
+            const uint32_t nc = chunk_alloc();
+            if (nc == NIL) return ERR_STACK;
+            LINK[nc] = c0;
+            c1 = nc;
+            *tp = (TopT)(nc << 4 | ((uint32_t)n - n1 - 1));
+        } else {
+            c1 = NIL;
+            *tp = (TopT)(c0 << 4 | (s0 + n1 - 1));
+        }
+        n_entries += n;
+        return 0;
+    };
 
     // result of the current read; the lane then claims the next one
     auto finish = [&](int code) __attribute__((always_inline)) {
         int n = code;
         if (n == ERR_HEAVY) {
-            a.heavy_list[atomicAdd(a.heavy_n, 1)] = rid;
+            CA(heavy_list)[atomicAdd(CA(heavy_n), 1)] = rid;
         } else {
             if (n > 0) {
-                const unsigned long long b0 = atomicAdd(a.out_top, (unsigned long long)n);
-                if (b0 + (unsigned long long)n > a.out_cap) {
+                const unsigned long long b0 = atomicAdd(CA(out_top), (unsigned long long)n);
+                if (b0 + (unsigned long long)n > CA(out_cap)) {
                     n = ERR_OUT;
                 } else {
-                    for (int j = 0; j < n; ++j) a.out_rec[b0 + j] = REC[j];
-                    a.out_off[rid] = b0;
+                    for (int j = 0; j < n; ++j) CA(out_rec)[b0 + j] = REC[j];
+                    CA(out_off)[rid] = b0;
                 }
             }
             if (n == ERR_STACK || n == ERR_ALN || n == ERR_OUT) {
-                const int f = atomicAdd(a.fail_n, 1);
-                a.fail_list[f] = rid;
-                atomicOr(a.fail_kind + (n == ERR_OUT ? 0 : n == ERR_ALN ? 1 : 2), 1);
+                const int f = atomicAdd(CA(fail_n), 1);
+                CA(fail_list)[f] = rid;
+                atomicOr(CA(fail_kind) + (n == ERR_OUT ? 0 : n == ERR_ALN ? 1 : 2), 1);
             }
-            a.out_n[rid] = n;
-            a.out_pops[rid] = pops;
-            if (TRACE) a.out_trace[rid] = trace;
-            if (a.out_bytes) a.out_bytes[rid] = obytes;
-            if (n != ERR_STACK && n != ERR_ALN && n != ERR_OUT) atomicAdd(a.algo_bytes, (unsigned long long)obytes);
+            CA(out_n)[rid] = n;
+            CA(out_pops)[rid] = pops;
+            if (TRACE) CA(out_trace)[rid] = trace;
+            if (CA(out_bytes)) CA(out_bytes)[rid] = obytes;
+            if (n != ERR_STACK && n != ERR_ALN && n != ERR_OUT) atomicAdd(CA(algo_bytes), (unsigned long long)obytes);
         }
-        if (n_own > a.keep) {
-            gran_push(a.ghead, a.gnext, OWN, a.keep, n_own);
-            n_own = a.keep;
+        if (n_own > CA(keep)) {
+            gran_push(CA(ghead), CA(gnext), OWN, CA(keep), n_own);
+            n_own = CA(keep);
         }
         rq_lf = rq_ent = rq_pf = rq_w = false;
         c1p = c2p = nullptr;
@@ -515,7 +560,7 @@ __global__ void __launch_bounds__(128) k_search(KArgs a)
         }
         if (do_add) {
             shadow((int)(uint32_t)(l - k + 1), e_ldp(e));
-            if (n_aln == a.rec_cap) return ERR_ALN;
+            if (n_aln == CA(rec_cap)) return ERR_ALN;
             const int ins = e_ins(e), del = go + ge - ins;
             galn_aln_rec_t r;
             r.packed = ((uint64_t)(mm & 0xff)) | ((uint64_t)(go & 0xff) << 8) | ((uint64_t)(ge & 0xff) << 16) |
@@ -534,6 +579,9 @@ __global__ void __launch_bounds__(128) k_search(KArgs a)
     // loaded first (NOFIN), a popped entry is a hit (HITP, at (hk, hl)), or the search ends
     // (the finish code). have_e: `e` was just loaded and not yet evaluated.
     auto pop_loop = [&](bool have_e) __attribute__((always_inline)) -> int {
+        uint32_t ntop = NIL;      // the lowest bucket's top after the last pop from memory
+        bool popped_mem = have_e;  // ntop is known
+        if (have_e) ntop = ld_ntop;
         for (;;) {
             if (!have_e) {
                 if (n_entries == 0 || n_entries > max_entries) return n_aln;
@@ -541,21 +589,28 @@ __global__ void __launch_bounds__(128) k_search(KArgs a)
                     e = cur;
                     e_score = cur_score;
                     has_cur = false;
+                    popped_mem = false;
                 } else {
+                    popped_mem = true;
                     if (!bm_any()) return ERR_INTERNAL;  // entries counted but none held: never expected
                     const int s = bm_low();
                     TopT *tp = tops + s * tstride;
                     const uint32_t t = *tp;
                     const uint32_t chunk = t >> 4, idx = t & 15u;
+                    // the bucket's new top: the lowest bucket's top unless the bucket empties
                     if (idx > 0) {
-                        *tp = (TopT)(t - 1);
+                        ntop = t - 1;
+                        *tp = (TopT)ntop;
                     } else {
                         const uint32_t p = LINK[chunk];
                         chunk_free(chunk);  // its entries stay readable until a later push
-                        if (p != NIL)
-                            *tp = (TopT)(p << 4 | (CH - 1));
-                        else
+                        if (p != NIL) {
+                            ntop = p << 4 | (CH - 1);
+                            *tp = (TopT)ntop;
+                        } else {
                             bm_clear(s);
+                            ntop = bm_any() ? tops[bm_low() * tstride] : NIL;
+                        }
                     }
                     e_score = s;
                     if (pf_valid && pf_addr == t) {
@@ -564,15 +619,13 @@ __global__ void __launch_bounds__(128) k_search(KArgs a)
                     } else {
                         ent_addr = t;
                         rq_ent = true;
+                        ld_ntop = ntop;
                         phase = PH_POP;
                         // and the entry now on top of the lowest bucket
-                        if (bm_any()) {
-                            const uint32_t t2 = tops[bm_low() * tstride];
-                            if (!pf_valid || pf_addr != t2) {
-                                pf_addr = t2;
-                                pf_valid = false;
-                                rq_pf = true;
-                            }
+                        if (ntop != NIL && (!pf_valid || pf_addr != ntop)) {
+                            pf_addr = ntop;
+                            pf_valid = false;
+                            rq_pf = true;
                         }
                         return NOFIN;
                     }
@@ -615,14 +668,13 @@ __global__ void __launch_bounds__(128) k_search(KArgs a)
             c1p = q + (i0 - 1);
             c2p = i0 >= 2 ? q + (i0 - 2) : nullptr;
             phase = PH_EXP;
-            // and the entry on top of the lowest bucket, usually the next pop
-            if (bm_any()) {
-                const uint32_t t = tops[bm_low() * tstride];
-                if (!pf_valid || pf_addr != t) {
-                    pf_addr = t;
-                    pf_valid = false;
-                    rq_pf = true;
-                }
+            // and the entry on top of the lowest bucket, usually the next pop (after a pop from
+            // memory ntop is that entry; after the exact child in registers it is unchanged)
+            if (!popped_mem && bm_any()) ntop = tops[bm_low() * tstride];
+            if (ntop != NIL && (!pf_valid || pf_addr != ntop)) {
+                pf_addr = ntop;
+                pf_valid = false;
+                rq_pf = true;
             }
             return NOFIN;
         }
@@ -705,11 +757,11 @@ __global__ void __launch_bounds__(128) k_search(KArgs a)
     };
 
     auto init_read = [&](int pos) __attribute__((always_inline)) -> int {
-        rid = a.list ? a.list[pos] : a.list_base + pos;
-        len = a.qlen[rid];
-        q = a.query + a.qoff[rid];
-        const int b = a.batch[rid];
-        const galn_search_opt_t o = a.bopts[b];
+        rid = CA(list) ? CA(list)[pos] : CA(list_base) + pos;
+        len = CA(qlen)[rid];
+        q = CA(query) + CA(qoff)[rid];
+        const int b = CA(batch)[rid];
+        const galn_search_opt_t o = CA(bopts)[b];
         s_mm = o.s_mm;
         s_gapo = o.s_gapo;
         s_gape = o.s_gape;
@@ -721,8 +773,8 @@ __global__ void __launch_bounds__(128) k_search(KArgs a)
         max_gape = o.max_gape;
         msd = o.max_seed_diff;
         max_top2 = o.max_top2;
-        n_stacks = a.nstacks[b];
-        md_read = a.maxdiff[rid];
+        n_stacks = CA(nstacks)[b];
+        md_read = CA(maxdiff)[rid];
         seed_len = o.seed_len;
         seeding = seed_len < len;
         pops = 0;
@@ -752,13 +804,13 @@ __global__ void __launch_bounds__(128) k_search(KArgs a)
         if (phase == PH_CLAIM) {
             int pos;
             if (PERSIST) {
-                pos = atomicAdd(a.qnext, 1);
+                pos = atomicAdd(CA(qnext), 1);
             } else {
                 if (claimed) break;
                 pos = slot;
             }
             claimed = true;
-            if (pos >= a.n_list) break;
+            if (pos >= CA(n_list)) break;
             fin = init_read(pos);
         } else {
             // ---- loads, all issued before any is used ----
@@ -912,44 +964,66 @@ __global__ void __launch_bounds__(128) k_search(KArgs a)
                     } else if (qbase < 4 && ca[qbase] + 1 <= cb[qbase]) {
                         mask |= 512u;
                     }
+                    // Pushes, in bwt_match_gap's order: the gap children (bits 0-4) all score sg
+                    // and the mismatches sm, so each group takes consecutive slots of one bucket
+                    // (two groups with one score share it, gap children first); the exact child
+                    // (bit 8 with qbase <= 3, or bit 9) comes last with the parent's score.
+                    const uint32_t xbit = (qbase <= 3 ? 256u : 0u) | 512u;
+                    const uint32_t mx = mask & xbit;
+                    const uint32_t mg = mask & 31u, mmk = mask & 0x1e0u & ~xbit;
+                    const int sg = st == ST_M ? score_of(mm, go + 1, ge) : score_of(mm, go, ge + 1);
+                    const int sm = score_of(mm + 1, go, ge);
+                    const int ng = __builtin_popcount(mg), nm = __builtin_popcount(mmk);
+                    const bool merged = ng > 0 && nm > 0 && sg == sm;
+                    uint32_t gc0 = 0, gs0 = 0, gn1 = 0, gc1 = 0, mc0 = 0, ms0 = 0, mn1 = 0, mc1 = 0;
                     int err = 0;
-                    while (mask && !err) {
-                        const int bit = __builtin_ctz(mask);
-                        mask &= mask - 1;
-                        Ent ch;
-                        int sc;
-                        bool exact = false;
-                        if (bit == 0) {
-                            if (st == ST_M) {
-                                sc = score_of(mm, go + 1, ge);
-                                ch = mk_ent(k, l, i, i, mm, go + 1, ge, ins + 1, ST_I);
+                    if (ng + (merged ? nm : 0) > 0) err = reserve(sg, ng + (merged ? nm : 0), gc0, gs0, gn1, gc1);
+                    if (!err && nm > 0 && !merged) err = reserve(sm, nm, mc0, ms0, mn1, mc1);
+                    if (!err) {
+                        uint32_t rest = mg | mmk;
+                        int rg = 0, rm = merged ? ng : 0;
+                        while (rest) {
+                            const int bit = __builtin_ctz(rest);
+                            rest &= rest - 1;
+                            Ent ch;
+                            int r;
+                            bool in_g = bit <= 4 || merged;
+                            if (bit == 0) {
+                                const int go2 = st == ST_M ? go + 1 : go, ge2 = st == ST_M ? ge : ge + 1;
+                                ch = mk_ent(k, l, i, i, mm, go2, ge2, ins + 1, ST_I);
+                            } else if (bit <= 4) {
+                                const int j = bit - 1;
+                                const uint64_t kk = M.L2[j] + ca[j] + 1, ll = M.L2[j] + cb[j];
+                                const int go2 = st == ST_M ? go + 1 : go, ge2 = st == ST_M ? ge : ge + 1;
+                                ch = mk_ent(kk, ll, i + 1, i + 1, mm, go2, ge2, ins, ST_D);
                             } else {
-                                sc = score_of(mm, go, ge + 1);
-                                ch = mk_ent(k, l, i, i, mm, go, ge + 1, ins + 1, ST_I);
+                                const int c = (qbase + bit - 4) & 3;
+                                const uint64_t kk = M.L2[c] + ca[c] + 1, ll = M.L2[c] + cb[c];
+                                ch = mk_ent(kk, ll, i, i, mm + 1, go, ge, ins, ST_M);
                             }
-                        } else if (bit <= 4) {
-                            const int j = bit - 1;
-                            const uint64_t kk = M.L2[j] + ca[j] + 1, ll = M.L2[j] + cb[j];
-                            const int go2 = st == ST_M ? go + 1 : go, ge2 = st == ST_M ? ge : ge + 1;
-                            sc = score_of(mm, go2, ge2);
-                            ch = mk_ent(kk, ll, i + 1, i + 1, mm, go2, ge2, ins, ST_D);
-                        } else {
-                            const int j = bit - 4;  // 1..4, or 5 for the lone exact child
-                            const int c = bit == 9 ? qbase : (qbase + j) & 3;
-                            const int is_mm = bit != 9 && (j != 4 || qbase > 3);
-                            const uint64_t kk = M.L2[c] + ca[c] + 1, ll = M.L2[c] + cb[c];
-                            sc = score_of(mm + is_mm, go, ge);
-                            ch = mk_ent(kk, ll, i, is_mm ? i : 0, mm + is_mm, go, ge, ins, ST_M);
-                            exact = !is_mm;
+                            if (bit <= 4)
+                                r = rg++;
+                            else
+                                r = rm++;
+                            const uint32_t c0 = in_g ? gc0 : mc0, s0 = in_g ? gs0 : ms0, n1 = in_g ? gn1 : mn1,
+                                           c1 = in_g ? gc1 : mc1;
+                            const uint32_t slot_ = (uint32_t)r < n1 ? c0 * CH + s0 + (uint32_t)r : c1 * CH + ((uint32_t)r - n1);
+                            CHK[slot_] = ch;
                         }
-                        if (exact && a.fuse) {
-                            // pushed last with the parent's score: the next pop; kept in registers
-                            cur = ch;
-                            cur_score = sc;
-                            has_cur = true;
-                            ++n_entries;
-                        } else {
-                            err = push_mem(sc, ch);
+                        if (mx) {
+                            const uint64_t kk = M.L2[qbase] + ca[qbase] + 1, ll = M.L2[qbase] + cb[qbase];
+                            const Ent ch = mk_ent(kk, ll, i, 0, mm, go, ge, ins, ST_M);
+                            if (a.fuse) {
+                                // pushed last with the parent's score: the next pop; kept in registers
+                                cur = ch;
+                                cur_score = e_score;
+                                has_cur = true;
+                                ++n_entries;
+                            } else {
+                                uint32_t c0, s0, n1, c1;
+                                err = reserve(e_score, 1, c0, s0, n1, c1);
+                                if (!err) CHK[n1 ? c0 * CH + s0 : c1 * CH] = ch;
+                            }
                         }
                     }
                     if (err) {
