@@ -101,6 +101,21 @@ def cpu_baseline(w, res, sample: int, threads: int, workdir: str) -> dict:
             "records_equal_gpu": agree}
 
 
+TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "k_search_k_heavy_adna10M_traffic.md")
+
+
+def measured_traffic(kernel: str, args) -> int | None:
+    """HBM bytes per launch of `kernel` measured by the FETCH_SIZE / WRITE_SIZE passes recorded
+    in profiles/ (scaled by a device copy of known size), for the default workload only."""
+    if args.reads != 10_000_000 or args.ref_bp != 100_000_000 or not os.path.exists(TRAFFIC_PROFILE):
+        return None
+    for ln in open(TRAFFIC_PROFILE):
+        f = ln.split()
+        if len(f) == 3 and f[0] == "traffic_bytes_per_launch" and f[1] == kernel:
+            return int(f[2])
+    return None
+
+
 def serving_sweep(w, index, nv, loads, seconds: float, max_req: int, seed: int = 7) -> dict:
     """Serving loop under Poisson arrivals. Requests are slices of the workload's reads of
     log-uniform size (1000 .. max_req reads), each searched with BWA's default options or the
@@ -276,12 +291,22 @@ def main() -> int:
 
     ms_b = 1e3 * sum(tb) / len(tb)
     value = w.n / (ms_b / 1e3)
-    # dominant kernel: the batched form's first-tier persistent launch
-    first = [l for l in lb[-1] if l["tier"] == 0]
-    k_ms = sum(l["ms"] for l in first) / len(first)
-    k_bytes = sum(l["algo_bytes"] for l in first) / len(first)
+    # per kernel of the last timed search: k_search (tiers 0-9) and k_heavy (tiers 10-)
+    per = {}
+    for l in lb[-1]:
+        kname = "k_search" if l["tier"] < 10 else "k_heavy"
+        d = per.setdefault(kname, {"ms": 0.0, "algo_bytes": 0, "launches": 0})
+        d["ms"] += l["ms"]
+        d["algo_bytes"] += l["algo_bytes"]
+        d["launches"] += 1
+    # the dominant kernel: the one with the most time; its first-tier launch is the unit
+    dom = max(per, key=lambda k: per[k]["ms"])
+    first = [l for l in lb[-1] if l["tier"] == (0 if dom == "k_search" else 10)]
+    k_ms = first[0]["ms"]
+    k_bytes = first[0]["algo_bytes"]
     achieved = k_bytes / (k_ms * 1e-3) / 1e9
     total_algo = int(res_b.occ_bytes.sum())
+    traffic = measured_traffic(dom, args)
     out = {
         "metric": METRIC,
         "value": value,
@@ -303,11 +328,11 @@ def main() -> int:
                   "ms_per_step": 1e3 * sum(tp) / len(tp) if tp else None, "steps": len(tp)},
         "batched_over_plain": (sum(tp) / len(tp)) / (sum(tb) / len(tb)) if tp else None,
         "parity": {"plain_eq_batched": same},
-        "kernel": {"name": "k_search<persistent>", "first_tier_ms": k_ms,
+        "kernel": {"dominant": dom, "launch_ms": k_ms, "launch_algo_bytes": k_bytes, "per_kernel": per,
                    "all_launches_ms": sum(l["ms"] for l in lb[-1]), "launches": len(lb[-1]),
-                   "algo_bytes_first_tier": k_bytes, "algo_bytes_total": total_algo},
+                   "algo_bytes_total": total_algo},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None},
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic},
         "ceiling": {"kind": "pointer chase, 64-byte random lines", "ws_bytes": ws, "GB/s": ceiling_gbs,
                     "frac": achieved / ceiling_gbs if ceiling_gbs else None},
     }

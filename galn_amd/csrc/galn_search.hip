@@ -2324,7 +2324,7 @@ int run_retries(galn_hip_readset *rs, uint32_t flags, int fi, int32_t n, Tier t,
         if ((r = run_tier(rs, flags, t, rs->d_fail[fi], 0, n, true, slots, fo, block, 1 + round, budget))) return r;
         int32_t nh = 0;
         if ((r = read_ctr(rs, 3, &nh))) return r;
-        if (nh > 0 && (r = run_heavy(rs, flags, rs->d_heavy, nh, 256 * (1 + round), fo, 1 + round, hwaves))) return r;
+        if (nh > 0 && (r = run_heavy(rs, flags, rs->d_heavy, nh, 256 * (1 + round), fo, 11 + round, hwaves))) return r;
         int32_t nf = 0;
         if ((r = read_ctr(rs, fo, &nf))) return r;
         fi = fo;

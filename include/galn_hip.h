@@ -176,7 +176,8 @@ int galn_hip_readset_results(galn_hip_readset *rs, galn_hip_out *out);
 /* Time of the last search's kernels in ms, measured with HIP events on the
  * search stream (first tier / all tiers). */
 int galn_hip_readset_kernel_ms(const galn_hip_readset *rs, double *first_tier_ms, double *total_ms);
-/* Per kernel launch of the last search: tier (0 = first), duration in ms (HIP events on the
+/* Per kernel launch of the last search: tier (0 = the first thread-per-read launch, 1.. its
+ * re-runs; 10 = the wave-per-read launch, 11.. its re-runs), duration in ms (HIP events on the
  * search stream) and the algorithmic index bytes of the reads it finished. Returns the number
  * of launches (fills at most `cap`). */
 int galn_hip_readset_launches(const galn_hip_readset *rs, int cap, int32_t *tier, double *ms, uint64_t *algo_bytes);
