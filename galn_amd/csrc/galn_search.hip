@@ -1081,7 +1081,7 @@ constexpr int H_STEPS = 32;    // chain steps per lane per round
 constexpr int H_PUSH = 128;    // logged pushes per lane per round
 constexpr int H_CAP = 4096;    // pushes committed per round
 constexpr int H_KEEP = 64;     // granules a wave keeps between reads
-constexpr int H_PIECES = MAXB + 16;
+constexpr int H_PIECES = 48;    // segment pieces one round may add (three buckets)
 constexpr uint32_t GENT = (uint32_t)GRAN * CH;  // entries per granule
 
 enum : int { K_END = 0, K_HIT = 1, K_TRUNC = 2, K_ESCORE = 3 };
@@ -1102,7 +1102,7 @@ struct HArgs {
     int32_t n_slots;
     uint8_t *scratch;
     uint64_t slot_bytes;
-    uint32_t rec_off, own_off, plog_off, slog_off, sinfo_off;
+    uint32_t rec_off, own_off, plog_off, slog_off;
     int32_t own_cap, rec_cap;
     Ent *pool;
     uint32_t *gnext;
@@ -1215,7 +1215,9 @@ __global__ void __launch_bounds__(64) k_heavy(HArgs a)
     uint64_t *SW = W + a.wlen;
     int32_t *lcommit = reinterpret_cast<int32_t *>(SW + a.swlen);  // per chain, for the trace hash
     int32_t *lnstart = lcommit + HW;
-    uint8_t *const PSC = reinterpret_cast<uint8_t *>(lnstart + HW) + threadIdx.x * H_PUSH;  // bucket of each logged push
+    uint32_t *const SINFO_ALL = reinterpret_cast<uint32_t *>(lnstart + HW);  // per chain step: pushes | bytes << 8
+    uint8_t *const PSC = reinterpret_cast<uint8_t *>(SINFO_ALL + HW * H_STEPS) + threadIdx.x * H_PUSH;  // bucket of each logged push
+    uint8_t *const QL = reinterpret_cast<uint8_t *>(SINFO_ALL + HW * H_STEPS) + HW * H_PUSH;  // the read's query
 
     const int lane = threadIdx.x;
     if ((int)blockIdx.x >= a.n_slots) return;
@@ -1224,7 +1226,6 @@ __global__ void __launch_bounds__(64) k_heavy(HArgs a)
     uint32_t *const OWN = reinterpret_cast<uint32_t *>(base + a.own_off);
     Ent *const PLOG = reinterpret_cast<Ent *>(base + a.plog_off) + lane * H_PUSH;
     Ent *const SLOG_ALL = reinterpret_cast<Ent *>(base + a.slog_off);
-    uint32_t *const SINFO_ALL = reinterpret_cast<uint32_t *>(base + a.sinfo_off);
     Ent *const SLOG = SLOG_ALL + lane * H_STEPS;
     uint32_t *const SINFO = SINFO_ALL + lane * H_STEPS;
     Ent *const POOL = a.pool;
@@ -1243,7 +1244,9 @@ __global__ void __launch_bounds__(64) k_heavy(HArgs a)
         // ---- read set-up (lane 0), widths (lanes 0 and 1) ----
         const int rid = a.list[pos];
         const int len = a.qlen[rid];
-        const uint8_t *q = a.query + a.qoff[rid];
+        const uint8_t *const qg = a.query + a.qoff[rid];
+        const uint8_t *const q = QL;
+        for (int j = lane; j < len && j < a.wlen; j += HW) QL[j] = qg[j];
         if (lane == 0) {
             const int b0 = a.batch[rid];
             const galn_search_opt_t o = a.bopts[b0];
@@ -2042,7 +2045,7 @@ int resident_slots(galn_hip_readset *rs, int block, bool trace, int waves_per_cu
 
 struct HLayout {
     uint64_t slot_bytes;
-    uint32_t rec_off, own_off, plog_off, slog_off, sinfo_off;
+    uint32_t rec_off, own_off, plog_off, slog_off;
 };
 
 HLayout make_hlayout(int rec_cap, int own_cap)
@@ -2057,8 +2060,6 @@ HLayout make_hlayout(int rec_cap, int own_cap)
     off += (size_t)HW * H_PUSH * sizeof(Ent);
     L.slog_off = (uint32_t)off;
     off += (size_t)HW * H_STEPS * sizeof(Ent);
-    L.sinfo_off = (uint32_t)off;
-    off += align_up((size_t)HW * H_STEPS * 4, 256);
     if ((off / 256) % 2 == 0) off += 256;
     L.slot_bytes = off;
     return L;
@@ -2069,7 +2070,8 @@ size_t heavy_lds_bytes(int buckets, int wlen, int swlen)
     size_t b = (sizeof(HShared) + 15) & ~(size_t)15;
     b += (size_t)buckets * 12;
     b = (b + 15) & ~(size_t)15;
-    b += (size_t)H_PIECES * 16 + (size_t)(wlen + swlen) * 8 + 2 * HW * 4 + (size_t)HW * H_PUSH;
+    b += (size_t)H_PIECES * 16 + (size_t)(wlen + swlen) * 8 + 2 * HW * 4 + (size_t)HW * H_STEPS * 4 +
+         (size_t)HW * H_PUSH + (size_t)wlen;
     return b;
 }
 
@@ -2214,7 +2216,6 @@ int run_heavy(galn_hip_readset *rs, uint32_t flags, const int32_t *list, int n, 
     a.own_off = L.own_off;
     a.plog_off = L.plog_off;
     a.slog_off = L.slog_off;
-    a.sinfo_off = L.sinfo_off;
     a.own_cap = own_cap;
     a.rec_cap = rec_cap;
     a.pool = idx->d_pool_chk;
