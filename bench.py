@@ -260,6 +260,9 @@ def main() -> int:
 
     index = nv.Index(w.words, w.meta, 0)
     rs = w.readset(index)
+    log("autotuning the batched form on this read set")
+    tuned = rs.autotune()
+    log(f"autotuned: {tuned}")
     log("warm-up")
     for _ in range(max(args.warmup, 0)):
         rs.search(nv.FORM_BATCHED)
@@ -323,6 +326,7 @@ def main() -> int:
         "config": {"workload": "batched aDNA mixed length: 100 Mbp reference (41% GC, 10% duplicated segments, 1% N), "
                                "10M reads 30-150 bp (median 45), aln -l 1024 -n 0.01 -o 2",
                    "ref_bp": args.ref_bp, "reads": w.n, "aln_opts": " ".join(w.opts), "form": "batched",
+                   "autotuned": tuned,
                    "index_bytes": ws},
         "plain": {"value": w.n / (sum(tp) / len(tp)) if tp else None,
                   "ms_per_step": 1e3 * sum(tp) / len(tp) if tp else None, "steps": len(tp)},

@@ -33,6 +33,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <atomic>
 #include <cstdarg>
 #include <cstdint>
@@ -1925,6 +1926,9 @@ struct galn_hip_readset {
     uint64_t cap_q = 0;
     // last search: one record per kernel launch, timed with events on the search stream
     double first_ms = 0, total_ms = 0;
+    // the batched form's tuning chosen by galn_hip_readset_autotune (used when a search passes none)
+    bool has_tuned = false;
+    galn_hip_tune_t tuned{};
     uint32_t last_flags = 0;
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
     struct Launch {
@@ -2334,8 +2338,13 @@ int run_retries(galn_hip_readset *rs, uint32_t flags, int fi, int32_t n, Tier t,
     return 0;
 }
 
+// Pops after which the batched form hands a read to the wave-per-read engine when no tuning is
+// given: measured best on the 10M-read aDNA workload (512: 2.67 s, 1024: 2.85 s, 4096: 3.34 s).
+constexpr int DEFAULT_HEAVY_POPS = 512;
+
 int readset_search(galn_hip_readset *rs, int form, uint32_t flags, const galn_hip_tune_t *tune)
 {
+    if (!tune && rs->has_tuned) tune = &rs->tuned;
     galn_hip_index *idx = rs->idx;
     HIP_OK(hipSetDevice(idx->device));
     rs->last_flags = flags;
@@ -2364,7 +2373,7 @@ int readset_search(galn_hip_readset *rs, int form, uint32_t flags, const galn_hi
         slots = resident_slots(rs, block, trace, tune ? tune->waves_per_cu : 0);
         const bool sorted = !tune || tune->sort_by_cost != 0;
         const int hp = tune ? tune->heavy_pops : 0;
-        budget = (hp < 0 || !heavy_ok(rs)) ? 0 : (uint64_t)(hp > 0 ? hp : 4096);
+        budget = (hp < 0 || !heavy_ok(rs)) ? 0 : (uint64_t)(hp > 0 ? hp : DEFAULT_HEAVY_POPS);
         hwaves = heavy_waves(rs, flags);
         r = run_tier(rs, flags, t0, sorted ? rs->d_sorted : nullptr, 0, rs->n, true, slots, 0, block, 0, budget);
         int32_t nh = 0;
@@ -2642,6 +2651,50 @@ int galn_hip_readset_search(galn_hip_index *idx, galn_hip_readset *rs, int form,
     if (!idx || !rs || rs->idx != idx) return fail(GALN_HIP_E_ARG, "readset_search: bad handle");
     if (rs->n == 0) return 0;
     return readset_search(rs, form, flags, tune);
+}
+
+int galn_hip_readset_autotune(galn_hip_index *idx, galn_hip_readset *rs, uint32_t flags, galn_hip_tune_t *out)
+{
+    if (!idx || !rs || rs->idx != idx) return fail(GALN_HIP_E_ARG, "readset_autotune: bad handle");
+    rs->has_tuned = false;
+    galn_hip_tune_t best{};
+    best.sort_by_cost = 1;
+    if (rs->n > 0) {
+        double best_ms = 0;
+        auto trial = [&](const galn_hip_tune_t &t) -> int {
+            const auto t0 = std::chrono::steady_clock::now();
+            const int r = readset_search(rs, 1, flags, &t);
+            if (r) return r;
+            const double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+            if (best_ms == 0 || ms < best_ms) {
+                best_ms = ms;
+                best = t;
+            }
+            return 0;
+        };
+        // the thread-vs-wave cutoff first (when the wave engine may run), then the workgroup size
+        galn_hip_tune_t t{};
+        t.sort_by_cost = 1;
+        t.block_threads = 128;
+        const int cut[] = {256, 512, 1024, 2048, 4096};
+        int r;
+        if (heavy_ok(rs)) {
+            for (int c : cut) {
+                t.heavy_pops = c;
+                if ((r = trial(t))) return r;
+            }
+        } else {
+            t.heavy_pops = -1;
+            if ((r = trial(t))) return r;
+        }
+        t = best;
+        t.block_threads = 64;
+        if ((r = trial(t))) return r;
+    }
+    rs->tuned = best;
+    rs->has_tuned = true;
+    if (out) *out = best;
+    return 0;
 }
 
 int galn_hip_readset_results(galn_hip_readset *rs, galn_hip_out *out)

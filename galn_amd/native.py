@@ -27,7 +27,7 @@ HIP_SYMBOLS = (
     "galn_hip_abi_version", "galn_hip_get_limits", "galn_hip_last_error", "galn_hip_index_open",
     "galn_hip_index_close", "galn_hip_index_device", "galn_hip_search_plain", "galn_hip_search_batched",
     "galn_hip_readset_upload", "galn_hip_readset_free", "galn_hip_readset_search", "galn_hip_readset_results",
-    "galn_hip_readset_reload", "galn_hip_readset_kernel_ms", "galn_hip_readset_launches",
+    "galn_hip_readset_reload", "galn_hip_readset_autotune", "galn_hip_readset_kernel_ms", "galn_hip_readset_launches",
     "galn_hip_readset_engine_stats", "galn_hip_server_open", "galn_hip_server_submit", "galn_hip_server_wait",
     "galn_hip_server_result", "galn_hip_server_stats", "galn_hip_server_close", "galn_hip_gather_ceiling",
 )
@@ -114,6 +114,7 @@ def hip_lib():
         L.galn_hip_readset_upload.argtypes = [P, C.POINTER(Reads), C.POINTER(P)]
         L.galn_hip_readset_free.argtypes = [P]
         L.galn_hip_readset_reload.argtypes = [P, C.POINTER(Reads)]
+        L.galn_hip_readset_autotune.argtypes = [P, P, C.c_uint32, C.POINTER(Tune)]
         L.galn_hip_server_open.argtypes = [P, C.POINTER(ServerCfg), C.POINTER(P)]
         L.galn_hip_server_submit.argtypes = [P, C.POINTER(Reads), C.POINTER(C.c_uint64)]
         L.galn_hip_server_wait.argtypes = [P, C.c_uint64, C.c_int]
@@ -373,6 +374,14 @@ class ReadSet:
                                                C.byref(tune) if tune is not None else None)
         if rc != 0:
             raise GalnError(f"readset_search failed ({rc}): {last_error()}")
+
+    def autotune(self, flags: int = 0) -> dict:
+        """galn_hip_readset_autotune: the chosen tuning (later searches without one use it)."""
+        t = Tune()
+        rc = hip_lib().galn_hip_readset_autotune(self.index.h, self.h, int(flags), C.byref(t))
+        if rc != 0:
+            raise GalnError(f"readset_autotune failed ({rc}): {last_error()}")
+        return {f: getattr(t, f) for f, _ in Tune._fields_}
 
     def launches(self) -> list[dict]:
         """[{tier, ms, algo_bytes}] for every kernel launch of the last search."""

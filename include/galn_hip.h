@@ -121,7 +121,7 @@ typedef struct {
     int32_t rec_per_lane;      /* record slots per lane in the first tier */
     int32_t sort_by_cost;      /* 1: claim reads longest-first */
     int32_t heavy_pops;        /* pops after which a read moves to the wave-per-read engine
-                                  (0: default, -1: never) */
+                                  (0: default 512, -1: never) */
 } galn_hip_tune_t;
 
 /* Serving loop configuration (0 = default). */
@@ -173,6 +173,12 @@ int galn_hip_readset_reload(galn_hip_readset *rs, const galn_hip_reads *reads);
 int galn_hip_readset_search(galn_hip_index *idx, galn_hip_readset *rs, int form, uint32_t flags,
                             const galn_hip_tune_t *tune);
 int galn_hip_readset_results(galn_hip_readset *rs, galn_hip_out *out);
+/* Autotuner of the batched form: searches the read set once per candidate -- the pops after
+ * which a read moves to the wave-per-read engine (256 .. 4096, when the options allow that
+ * engine), then 64- or 128-thread workgroups for the best -- and keeps the fastest (host clock
+ * around each search). Later batched searches of this read set that pass no tuning use it.
+ * The chosen tuning is copied to *out (may be NULL). Results are those of the last trial. */
+int galn_hip_readset_autotune(galn_hip_index *idx, galn_hip_readset *rs, uint32_t flags, galn_hip_tune_t *out);
 /* Time of the last search's kernels in ms, measured with HIP events on the
  * search stream (first tier / all tiers). */
 int galn_hip_readset_kernel_ms(const galn_hip_readset *rs, double *first_tier_ms, double *total_ms);
