@@ -1217,8 +1217,7 @@ __global__ void __launch_bounds__(64) k_heavy(HArgs a)
     int32_t *lcommit = reinterpret_cast<int32_t *>(SW + a.swlen);  // per chain, for the trace hash
     int32_t *lnstart = lcommit + HW;
     uint32_t *const SINFO_ALL = reinterpret_cast<uint32_t *>(lnstart + HW);  // per chain step: pushes | bytes << 8
-    uint8_t *const PSC = reinterpret_cast<uint8_t *>(SINFO_ALL + HW * H_STEPS) + threadIdx.x * H_PUSH;  // bucket of each logged push
-    uint8_t *const QL = reinterpret_cast<uint8_t *>(SINFO_ALL + HW * H_STEPS) + HW * H_PUSH;  // the read's query
+    uint8_t *const QL = reinterpret_cast<uint8_t *>(SINFO_ALL + HW * H_STEPS);  // the read's query
 
     const int lane = threadIdx.x;
     if ((int)blockIdx.x >= a.n_slots) return;
@@ -1238,6 +1237,7 @@ __global__ void __launch_bounds__(64) k_heavy(HArgs a)
         F.n_own = 0;  // granules this wave holds; a read reuses them before taking more
     }
     for (;;) {
+        const long long t_read = a.stats ? wall_clock64() : 0;
         if (lane == 0) F.claim = atomicAdd(a.qnext, 1);
         __syncthreads();
         const int pos = F.claim;
@@ -1317,8 +1317,15 @@ __global__ void __launch_bounds__(64) k_heavy(HArgs a)
             }
         }
         __syncthreads();
+        if (a.stats && lane == 0) {
+            F.st[12] += (unsigned long long)(wall_clock64() - t_read);
+            F.st[15] += 1;
+        }
         // ---- rounds ----
+        long long t_round_end = 0;  // with stats: end of the last round's commit
         while (F.status == 0) {
+            const long long t_round = a.stats ? wall_clock64() : 0;
+            if (a.stats && lane == 0 && t_round_end) F.st[14] += (unsigned long long)(t_round - t_round_end);
             if (lane == 0) {
                 // a pending exact child is the top of the lowest bucket (the bucket it came from)
                 const int b = F.has_pend ? F.b : hbm_low(F.bm);
@@ -1361,11 +1368,13 @@ __global__ void __launch_bounds__(64) k_heavy(HArgs a)
             // ---- speculation: lane j runs the chain of the j-th entry from the top of bucket b ----
             const int b = F.b, K = F.K, hp = F.has_pend;
             const long long t_spec = a.stats ? wall_clock64() : 0;
+            if (a.stats && lane == 0) F.st[13] += (unsigned long long)(t_spec - t_round);
             const int s_mm = F.s_mm, s_gapo = F.s_gapo, s_gape = F.s_gape, mode = F.mode, max_diff = F.max_diff;
             const int n_stacks = F.n_stacks, max_gapo = F.max_gapo, max_gape = F.max_gape, ies = F.ies, mdo = F.mdo;
             const int msd = F.msd, seed_len = F.seed_len;
             const bool seeding = F.seeding != 0;
             int steps = 0, np = 0, kind = K_END;
+            uint64_t pcode = 0;  // bucket codes of the first 32 logged pushes
             uint64_t hk = 0, hl = 0;
             Ent nxt{0, 0};
             if (lane < K) {
@@ -1453,7 +1462,12 @@ __global__ void __launch_bounds__(64) k_heavy(HArgs a)
                                         return;
                                     }
                                     PLOG[np] = en;
-                                    PSC[np] = (uint8_t)s;
+                                    // its bucket as the commit numbers them: 0 b+s_mm, 1 b+s_gapo,
+                                    // 2 b+s_gape, 3 none of them (not possible with positive penalties)
+                                    if (np < 32) {
+                                        const int d = s - b;
+                                        pcode |= (uint64_t)(d == s_mm ? 0 : d == s_gapo ? 1 : d == s_gape ? 2 : 3) << (2 * np);
+                                    }
                                     ++np;
                                     ++sp;
                                 };
@@ -1618,11 +1632,18 @@ __global__ void __launch_bounds__(64) k_heavy(HArgs a)
             const int tg0 = b + s_mm, tg1 = b + s_gapo, tg2 = b + s_gape;
             int n0 = 0, n1 = 0, n2 = 0, bad = 0;
             for (int u = 0; u < ncp; ++u) {
-                const int sc = PSC[u];
-                if (sc == tg0) ++n0;
-                else if (sc == tg1) ++n1;
-                else if (sc == tg2) ++n2;
-                else bad = 1;
+                int x;
+                if (u < 32) {
+                    x = (int)(pcode >> (2 * u)) & 3;
+                } else {
+                    const Ent pu = PLOG[u];
+                    const int sc = e_mm(pu) * s_mm + e_go(pu) * s_gapo + e_ge(pu) * s_gape;  // its bucket
+                    x = sc == tg0 ? 0 : sc == tg1 ? 1 : sc == tg2 ? 2 : 3;
+                }
+                n0 += x == 0;
+                n1 += x == 1;
+                n2 += x == 2;
+                bad |= x == 3;
             }
             const int r0 = wave_excl_scan(n0), r1 = wave_excl_scan(n1), r2 = wave_excl_scan(n2);
             const int T0 = __shfl(r0 + n0, HW - 1), T1 = __shfl(r1 + n1, HW - 1), T2 = __shfl(r2 + n2, HW - 1);
@@ -1693,12 +1714,29 @@ __global__ void __launch_bounds__(64) k_heavy(HArgs a)
             const long long t_c2 = a.stats ? wall_clock64() : 0;
             {
                 uint32_t q0 = (uint32_t)r0, q1 = (uint32_t)r1, q2 = (uint32_t)r2;
-                for (int u = 0; u < ncp; ++u) {
-                    const int sc = PSC[u];
-                    const uint32_t r = sc == tg0 ? q0++ : sc == tg1 ? q1++ : q2++;
-                    int p = first_piece[sc];
-                    while (r >= pieces[p].y + pieces[p].z) ++p;
-                    POOL[pieces[p].w + (r - pieces[p].y)] = PLOG[u];
+                const int tgs[3] = {tg0, tg1, tg2};
+                // four log entries loaded together, then placed
+                for (int u0 = 0; u0 < ncp; u0 += 4) {
+                    Ent pu[4];
+#pragma unroll
+                    for (int t = 0; t < 4; ++t)
+                        if (u0 + t < ncp) pu[t] = PLOG[u0 + t];
+#pragma unroll
+                    for (int t = 0; t < 4; ++t) {
+                        const int u = u0 + t;
+                        if (u >= ncp) break;
+                        int x;
+                        if (u < 32) {
+                            x = (int)(pcode >> (2 * u)) & 3;
+                        } else {
+                            const int sc = e_mm(pu[t]) * s_mm + e_go(pu[t]) * s_gapo + e_ge(pu[t]) * s_gape;
+                            x = sc == tg0 ? 0 : sc == tg1 ? 1 : 2;  // (3 was ruled out above)
+                        }
+                        const uint32_t r = x == 0 ? q0++ : x == 1 ? q1++ : q2++;
+                        int p = first_piece[tgs[x]];
+                        while (r >= pieces[p].y + pieces[p].z) ++p;
+                        POOL[pieces[p].w + (r - pieces[p].y)] = pu[t];
+                    }
                 }
             }
             const int mxs = a.stats ? wave_max_i32(steps) : 0;
@@ -1738,6 +1776,7 @@ __global__ void __launch_bounds__(64) k_heavy(HArgs a)
                     F.st[4] += (unsigned long long)mxs;
                     F.st[5] += (unsigned long long)(t_commit - t_spec);
                     const long long t_end = wall_clock64();
+                    t_round_end = t_end;
                     F.st[6] += (unsigned long long)(t_end - t_commit);
                     F.st[7] += (unsigned long long)(t_c1 - t_commit);
                     F.st[8] += (unsigned long long)(t_c2 - t_c1);
@@ -2074,8 +2113,7 @@ size_t heavy_lds_bytes(int buckets, int wlen, int swlen)
     size_t b = (sizeof(HShared) + 15) & ~(size_t)15;
     b += (size_t)buckets * 12;
     b = (b + 15) & ~(size_t)15;
-    b += (size_t)H_PIECES * 16 + (size_t)(wlen + swlen) * 8 + 2 * HW * 4 + (size_t)HW * H_STEPS * 4 +
-         (size_t)HW * H_PUSH + (size_t)wlen;
+    b += (size_t)H_PIECES * 16 + (size_t)(wlen + swlen) * 8 + 2 * HW * 4 + (size_t)HW * H_STEPS * 4 + (size_t)wlen;
     return b;
 }
 

@@ -398,7 +398,8 @@ class ReadSet:
         v = np.zeros(16, np.uint64)
         hip_lib().galn_hip_readset_engine_stats(self.h, _p(v))
         keys = ("rounds", "chains", "chains_committed", "steps_committed", "critical_steps", "spec_ticks",
-                "commit_ticks", "plan_ticks", "alloc_ticks", "place_ticks", "update_ticks", "granules")
+                "commit_ticks", "plan_ticks", "alloc_ticks", "place_ticks", "update_ticks", "granules", "setup_ticks",
+                "start_ticks", "after_ticks", "reads")
         return {k: int(v[i]) for i, k in enumerate(keys)}
 
     def kernel_ms(self) -> tuple[float, float]:

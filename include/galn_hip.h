@@ -190,7 +190,8 @@ int galn_hip_readset_launches(const galn_hip_readset *rs, int cap, int32_t *tier
 /* Counters of the wave-per-read engine in the last search made with GALN_HIP_STATS: rounds,
  * chains started, chains committed, steps committed, longest chain per round (summed), then
  * clock ticks (100 MHz, summed over waves) in speculation, in commits, and in the commits'
- * parts: plan, piece allocation, placement, state update; granules taken; 0 x 4. */
+ * parts: plan, piece allocation, placement, state update; granules taken; ticks in read
+ * set-up (claim, widths), in round starts, between a commit and the next round (hits); reads. */
 int galn_hip_readset_engine_stats(galn_hip_readset *rs, uint64_t out[16]);
 
 /* Serving: the index stays resident, requests (each a galn_hip_reads with its own options)
