@@ -260,12 +260,15 @@ def main() -> int:
 
     index = nv.Index(w.words, w.meta, 0)
     rs = w.readset(index)
+    # the batched form: persistent kernel + device work queue, launches replayed as a captured
+    # hipGraph, tuning chosen by the library's autotuner on this read set
+    BATCHED_FLAGS = nv.GALN_HIP_GRAPH
     log("autotuning the batched form on this read set")
-    tuned = rs.autotune()
+    tuned = rs.autotune(BATCHED_FLAGS)
     log(f"autotuned: {tuned}")
     log("warm-up")
     for _ in range(max(args.warmup, 0)):
-        rs.search(nv.FORM_BATCHED)
+        rs.search(nv.FORM_BATCHED, BATCHED_FLAGS)
         rs.results(want_pops=False)
 
     def timed(form: int, steps: int):
@@ -274,8 +277,8 @@ def main() -> int:
         for s in range(steps):
             torch.cuda.synchronize()
             t0 = time.perf_counter()
-            rs.search(form)
-            res = rs.results(want_pops=False, want_bytes=(s == steps - 1))
+            rs.search(form, BATCHED_FLAGS if form == nv.FORM_BATCHED else 0)
+            res = rs.results(want_pops=(s == steps - 1), want_bytes=(s == steps - 1))
             torch.cuda.synchronize()
             times.append(time.perf_counter() - t0)
             launches.append(rs.launches())
@@ -310,6 +313,10 @@ def main() -> int:
     achieved = k_bytes / (k_ms * 1e-3) / 1e9
     total_algo = int(res_b.occ_bytes.sum())
     traffic = measured_traffic(dom, args)
+    # per-read backtracking (popped stack entries), log2 bins: [0], [1], [2,3], [4,7], ...
+    pops = res_b.pops.astype(np.int64)
+    bins = np.bincount(np.where(pops > 0, np.floor(np.log2(np.maximum(pops, 1))).astype(np.int64) + 1, 0))
+    pops_hist = {("0" if b == 0 else f"{1 << (b - 1)}-{(1 << b) - 1}"): int(c) for b, c in enumerate(bins) if c}
     out = {
         "metric": METRIC,
         "value": value,
@@ -332,6 +339,7 @@ def main() -> int:
                   "ms_per_step": 1e3 * sum(tp) / len(tp) if tp else None, "steps": len(tp)},
         "batched_over_plain": (sum(tp) / len(tp)) / (sum(tb) / len(tb)) if tp else None,
         "parity": {"plain_eq_batched": same},
+        "pops_per_read": {"mean": float(pops.mean()), "max": int(pops.max()), "log2_histogram": pops_hist},
         "kernel": {"dominant": dom, "launch_ms": k_ms, "launch_algo_bytes": k_bytes, "per_kernel": per,
                    "all_launches_ms": sum(l["ms"] for l in lb[-1]), "launches": len(lb[-1]),
                    "algo_bytes_total": total_algo},

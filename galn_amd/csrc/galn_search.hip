@@ -1099,6 +1099,7 @@ struct HArgs {
     const int32_t *nstacks;
     const int32_t *list;
     int32_t n_list;
+    const int32_t *n_list_dev;  // when set, the number of reads is read here (written by k_search)
     int32_t *qnext;
     int32_t n_slots;
     uint8_t *scratch;
@@ -1236,12 +1237,13 @@ __global__ void __launch_bounds__(64) k_heavy(HArgs a)
         for (int j = 0; j < 16; ++j) F.st[j] = 0;
         F.n_own = 0;  // granules this wave holds; a read reuses them before taking more
     }
+    const int n_list = a.n_list_dev ? *a.n_list_dev : a.n_list;
     for (;;) {
         const long long t_read = a.stats ? wall_clock64() : 0;
         if (lane == 0) F.claim = atomicAdd(a.qnext, 1);
         __syncthreads();
         const int pos = F.claim;
-        if (pos >= a.n_list) break;
+        if (pos >= n_list) break;
         // ---- read set-up (lane 0), widths (lanes 0 and 1) ----
         const int rid = a.list[pos];
         const int len = a.qlen[rid];
@@ -1884,6 +1886,13 @@ __global__ void __launch_bounds__(64) k_heavy(HArgs a)
     if (a.stats && lane < 16) atomicAdd(a.stats + lane, F.st[lane]);
 }
 
+// Fills n words with v: the search's counter and pool-head resets, as kernels so that a
+// captured graph holds no memset nodes.
+__global__ void k_fill_u32(uint32_t *p, uint32_t v, uint32_t n)
+{
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = v;
+}
+
 // ---------------------------------------------------------------------------
 // host side
 
@@ -1926,6 +1935,17 @@ struct galn_hip_index {
     unsigned long long *d_ghead = nullptr;
     uint32_t n_gran = 0;
     std::atomic<int> n_readsets{0};  // read sets not yet freed (they use the stream and pool)
+};
+
+struct GraphKey {
+    uint32_t flags;
+    int32_t block, slots, hwaves, n;
+    uint64_t budget;
+    uint32_t rec_cap, keep;
+    const int32_t *list;
+    const void *ptr[20];
+    uint64_t rec_cap_out, n_gran, scratch_bytes, hscratch_bytes;
+    int32_t max_len, seed_len, max_stacks, max_entries;
 };
 
 struct galn_hip_readset {
@@ -1977,6 +1997,10 @@ struct galn_hip_readset {
         uint64_t bytes;
     };
     std::vector<Launch> launches;
+    // the batched form's captured graph (GALN_HIP_GRAPH), its key and its launch records
+    hipGraphExec_t gexec = nullptr;
+    GraphKey gkey{};
+    std::vector<Launch> glaunches;
     std::vector<hipEvent_t> lev;           // 2 per launch, reused across searches
     unsigned long long *d_algo = nullptr;  // per-launch algorithmic byte counters
     static constexpr int MAX_LAUNCH = 256;
@@ -2067,6 +2091,20 @@ int launch(galn_hip_index *idx, const KArgs &a, int block, int grid, bool persis
     return persist ? launch_one<false, true>(idx, a, block, grid, buckets)
                    : launch_one<false, false>(idx, a, block, grid, buckets);
 }
+
+int fill_u32(hipStream_t st, void *p, uint32_t v, size_t bytes)
+{
+    const uint32_t n = (uint32_t)(bytes / 4);
+    hipLaunchKernelGGL(k_fill_u32, dim3(std::max(1u, std::min(64u, (n + 255) / 256))), dim3(256), 0, st,
+                       reinterpret_cast<uint32_t *>(p), v, n);
+    HIP_OK(hipGetLastError());
+    return 0;
+}
+#define FILL(p, v, bytes)                                                                              \
+    do {                                                                                               \
+        const int fr_ = fill_u32(idx->stream, (p), (v), (bytes));                                      \
+        if (fr_) return fr_;                                                                           \
+    } while (0)
 
 int buckets_of(const galn_hip_readset *rs) { return std::max(1, std::min(rs->max_stacks, MAXB)); }
 
@@ -2161,6 +2199,50 @@ KArgs base_args(galn_hip_readset *rs, uint32_t flags)
 
 // One launch over list positions [0, n_list) (read id = list[pos], or list_base + pos);
 // failures land in fail buffer `fo`.
+// The allocations a first-tier launch over `slots` lanes needs (run before a graph capture).
+int prepare_tier(galn_hip_readset *rs, const Tier &t, int slots)
+{
+    const int own_cap = (int)(((int64_t)(rs->max_entries + 16) / CH + MAXB + 2) / GRAN + 2);
+    const Layout L = make_layout(rs->max_len, rs->seed_len, t.rec_cap, own_cap);
+    int r = ensure_scratch(rs, (size_t)slots * L.slot_bytes);
+    if (r) return r;
+    return ensure_pool(rs->idx, (uint64_t)slots * (uint64_t)(t.keep + 12));
+}
+
+// The allocations of a wave-engine launch of `waves` waves.
+int prepare_heavy(galn_hip_readset *rs, int waves, int rec_cap)
+{
+    galn_hip_index *idx = rs->idx;
+    int r = ensure_pool(idx, (uint64_t)waves * 256);
+    if (r) return r;
+    const int own_cap = (int)std::min<uint64_t>(idx->n_gran, 32768);
+    const HLayout L = make_hlayout(rec_cap, own_cap);
+    if (rs->hscratch_bytes < (size_t)waves * L.slot_bytes) {
+        if (rs->d_hscratch) HIP_OK(hipFree(rs->d_hscratch));
+        rs->d_hscratch = nullptr;
+        rs->hscratch_bytes = 0;
+        HIP_OK(hipMalloc(&rs->d_hscratch, (size_t)waves * L.slot_bytes));
+        rs->hscratch_bytes = (size_t)waves * L.slot_bytes;
+    }
+    return 0;
+}
+
+// What a captured search graph holds: when any of it changes, the graph is captured again.
+void fill_key_ptrs(const galn_hip_readset *rs, GraphKey &k)
+{
+    const galn_hip_index *idx = rs->idx;
+    const void *p[] = {rs->d_query, rs->d_qoff, rs->d_qlen, rs->d_maxdiff, rs->d_batch, rs->d_nstacks, rs->d_bopts,
+                       rs->d_sorted, rs->d_out_n, rs->d_out_off, rs->d_pops, rs->d_trace, rs->d_bytes, rs->d_rec,
+                       rs->d_fail[0], rs->d_fail[1], rs->d_heavy, rs->d_scratch, rs->d_hscratch, idx->d_pool_chk};
+    static_assert(sizeof(p) == sizeof(k.ptr), "graph key pointers");
+    memcpy(k.ptr, p, sizeof p);
+    k.rec_cap_out = rs->rec_cap;
+    k.n_gran = idx->n_gran;
+    k.scratch_bytes = rs->scratch_bytes;
+    k.hscratch_bytes = rs->hscratch_bytes;
+    k.max_len = rs->max_len, k.seed_len = rs->seed_len, k.max_stacks = rs->max_stacks, k.max_entries = rs->max_entries;
+}
+
 int run_tier(galn_hip_readset *rs, uint32_t flags, const Tier &t, const int32_t *list, int list_base, int n_list,
              bool persist, int slots, int fo, int block, int tier_id, uint64_t pop_budget = 0)
 {
@@ -2170,10 +2252,9 @@ int run_tier(galn_hip_readset *rs, uint32_t flags, const Tier &t, const int32_t 
     slots = std::max(1, std::min(slots, n_list));
     const int own_cap = (int)(((int64_t)(rs->max_entries + 16) / CH + MAXB + 2) / GRAN + 2);
     Layout L = make_layout(rs->max_len, rs->seed_len, t.rec_cap, own_cap);
-    int r = ensure_scratch(rs, (size_t)slots * L.slot_bytes);
+    // scratch, and a pool for what the lanes keep plus room for the heavy reads in flight
+    int r = prepare_tier(rs, t, slots);
     if (r) return r;
-    // pool: what the lanes keep plus room for the heavy reads in flight
-    if ((r = ensure_pool(idx, (uint64_t)slots * (uint64_t)(t.keep + 12)))) return r;
     KArgs a = base_args(rs, flags);
     a.list = list;
     a.list_base = list_base;
@@ -2194,10 +2275,10 @@ int run_tier(galn_hip_readset *rs, uint32_t flags, const Tier &t, const int32_t 
     a.pop_budget = pop_budget;
     a.heavy_list = rs->d_heavy;
     a.heavy_n = rs->d_ctr + 3;
-    if (persist) HIP_OK(hipMemsetAsync(rs->d_ctr + 2, 0, sizeof(int32_t), idx->stream));
+    if (persist) FILL(rs->d_ctr + 2, 0, sizeof(int32_t));
     // an empty free stack (tag and granule all ones) and no granule handed out yet
-    HIP_OK(hipMemsetAsync(idx->d_ghead, 0xff, NSHARD * 8, idx->stream));
-    HIP_OK(hipMemsetAsync(idx->d_gbump, 0, 4, idx->stream));
+    FILL(idx->d_ghead, 0xffffffffu, NSHARD * 8);
+    FILL(idx->d_gbump, 0, 4);
     const int li = (int)rs->launches.size();
     if (li >= galn_hip_readset::MAX_LAUNCH) return fail(GALN_HIP_E_DEVICE, "too many launches in one search");
     while ((int)rs->lev.size() < 2 * (li + 1)) {
@@ -2221,19 +2302,13 @@ int run_heavy(galn_hip_readset *rs, uint32_t flags, const int32_t *list, int n, 
               int waves)
 {
     galn_hip_index *idx = rs->idx;
-    if (n <= 0) return 0;
-    waves = std::max(1, std::min(waves, n));
-    int r = ensure_pool(idx, (uint64_t)waves * 256);
+    const bool dev_count = n < 0;  // the count is the heavy-list length k_search leaves on the device
+    if (n == 0) return 0;
+    waves = std::max(1, dev_count ? waves : std::min(waves, n));
+    int r = prepare_heavy(rs, waves, rec_cap);
     if (r) return r;
     const int own_cap = (int)std::min<uint64_t>(idx->n_gran, 32768);
     const HLayout L = make_hlayout(rec_cap, own_cap);
-    if (rs->hscratch_bytes < (size_t)waves * L.slot_bytes) {
-        if (rs->d_hscratch) HIP_OK(hipFree(rs->d_hscratch));
-        rs->d_hscratch = nullptr;
-        rs->hscratch_bytes = 0;
-        HIP_OK(hipMalloc(&rs->d_hscratch, (size_t)waves * L.slot_bytes));
-        rs->hscratch_bytes = (size_t)waves * L.slot_bytes;
-    }
     const int wlen = rs->max_len + 1;
     const int swlen = (rs->seed_len > 0 && rs->seed_len < rs->max_len) ? rs->seed_len + 1 : 1;
     const size_t lds = heavy_lds_bytes(buckets_of(rs), wlen, swlen);
@@ -2249,7 +2324,8 @@ int run_heavy(galn_hip_readset *rs, uint32_t flags, const int32_t *list, int n, 
     a.bopts = rs->d_bopts;
     a.nstacks = rs->d_nstacks;
     a.list = list;
-    a.n_list = n;
+    a.n_list = dev_count ? 0 : n;
+    a.n_list_dev = dev_count ? rs->d_ctr + 3 : nullptr;
     a.qnext = rs->d_ctr + 2;
     a.n_slots = waves;
     a.scratch = rs->d_hscratch;
@@ -2280,9 +2356,9 @@ int run_heavy(galn_hip_readset *rs, uint32_t flags, const int32_t *list, int n, 
     a.wlen = wlen;
     a.swlen = swlen;
     a.stats = (flags & GALN_HIP_STATS) ? rs->d_stats : nullptr;
-    HIP_OK(hipMemsetAsync(rs->d_ctr + 2, 0, sizeof(int32_t), idx->stream));
-    HIP_OK(hipMemsetAsync(idx->d_ghead, 0xff, NSHARD * 8, idx->stream));
-    HIP_OK(hipMemsetAsync(idx->d_gbump, 0, 4, idx->stream));
+    FILL(rs->d_ctr + 2, 0, sizeof(int32_t));
+    FILL(idx->d_ghead, 0xffffffffu, NSHARD * 8);
+    FILL(idx->d_gbump, 0, 4);
     const int li = (int)rs->launches.size();
     if (li >= galn_hip_readset::MAX_LAUNCH) return fail(GALN_HIP_E_DEVICE, "too many launches in one search");
     while ((int)rs->lev.size() < 2 * (li + 1)) {
@@ -2362,8 +2438,8 @@ int run_retries(galn_hip_readset *rs, uint32_t flags, int fi, int32_t n, Tier t,
         }
         slots = std::min(slots, std::max(block, (int)n));
         const int fo = fi ^ 1;
-        HIP_OK(hipMemsetAsync(rs->d_ctr + fo, 0, sizeof(int32_t), idx->stream));
-        HIP_OK(hipMemsetAsync(rs->d_ctr + 3, 0, 4 * sizeof(int32_t), idx->stream));
+        FILL(rs->d_ctr + fo, 0, sizeof(int32_t));
+        FILL(rs->d_ctr + 3, 0, 4 * sizeof(int32_t));
         if ((r = run_tier(rs, flags, t, rs->d_fail[fi], 0, n, true, slots, fo, block, 1 + round, budget))) return r;
         int32_t nh = 0;
         if ((r = read_ctr(rs, 3, &nh))) return r;
@@ -2387,11 +2463,19 @@ int readset_search(galn_hip_readset *rs, int form, uint32_t flags, const galn_hi
     HIP_OK(hipSetDevice(idx->device));
     rs->last_flags = flags;
     rs->launches.clear();
-    HIP_OK(hipMemsetAsync(rs->d_algo, 0, sizeof(unsigned long long) * galn_hip_readset::MAX_LAUNCH, idx->stream));
-    HIP_OK(hipMemsetAsync(rs->d_stats, 0, 16 * 8, idx->stream));
-    HIP_OK(hipMemsetAsync(rs->d_top, 0, sizeof(unsigned long long), idx->stream));
-    HIP_OK(hipMemsetAsync(rs->d_ctr, 0, 8 * sizeof(int32_t), idx->stream));
     const bool trace = (flags & GALN_HIP_TRACE) != 0;
+    const bool graph = form != 0 && (flags & GALN_HIP_GRAPH) != 0;
+    auto prologue = [&]() -> int {
+        FILL(rs->d_algo, 0, sizeof(unsigned long long) * galn_hip_readset::MAX_LAUNCH);
+        FILL(rs->d_stats, 0, 16 * 8);
+        FILL(rs->d_top, 0, sizeof(unsigned long long));
+        FILL(rs->d_ctr, 0, 8 * sizeof(int32_t));
+        return 0;
+    };
+    if (!graph) {
+        const int pr = prologue();
+        if (pr) return pr;
+    }
     int block = pick_block(rs);
     Tier t0{64, 4};
     if (tune && tune->rec_per_lane > 0) t0.rec_cap = tune->rec_per_lane;
@@ -2413,10 +2497,52 @@ int readset_search(galn_hip_readset *rs, int form, uint32_t flags, const galn_hi
         const int hp = tune ? tune->heavy_pops : 0;
         budget = (hp < 0 || !heavy_ok(rs)) ? 0 : (uint64_t)(hp > 0 ? hp : DEFAULT_HEAVY_POPS);
         hwaves = heavy_waves(rs, flags);
-        r = run_tier(rs, flags, t0, sorted ? rs->d_sorted : nullptr, 0, rs->n, true, slots, 0, block, 0, budget);
-        int32_t nh = 0;
-        if (!r && budget && !(r = read_ctr(rs, 3, &nh)) && nh > 0)
-            r = run_heavy(rs, flags, rs->d_heavy, nh, 256, 0, 10, hwaves);
+        const int32_t *list = sorted ? rs->d_sorted : nullptr;
+        if (graph) {
+            // the prologue, the persistent launch and the wave-engine launch (its read count read
+            // on the device) as one captured graph, replayed while nothing it holds has changed
+            GraphKey key;
+            memset(&key, 0, sizeof key);  // padding included: keys are compared bytewise
+            key.flags = flags, key.block = block, key.slots = slots, key.hwaves = hwaves, key.n = rs->n;
+            key.budget = budget, key.rec_cap = (uint32_t)t0.rec_cap, key.keep = (uint32_t)t0.keep, key.list = list;
+            if ((r = prepare_tier(rs, t0, slots)) || (budget && (r = prepare_heavy(rs, hwaves, 256)))) return r;
+            while (rs->lev.size() < 4) {
+                hipEvent_t e;
+                HIP_OK(hipEventCreate(&e));
+                rs->lev.push_back(e);
+            }
+            fill_key_ptrs(rs, key);
+            if (!rs->gexec || memcmp(&key, &rs->gkey, sizeof key) != 0) {
+                if (rs->gexec) HIP_OK(hipGraphExecDestroy(rs->gexec));
+                rs->gexec = nullptr;
+                HIP_OK(hipStreamBeginCapture(idx->stream, hipStreamCaptureModeThreadLocal));
+                r = prologue();
+                if (!r) r = run_tier(rs, flags, t0, list, 0, rs->n, true, slots, 0, block, 0, budget);
+                if (!r && budget) r = run_heavy(rs, flags, rs->d_heavy, -1, 256, 0, 10, hwaves);
+                hipGraph_t g = nullptr;
+                const hipError_t ec = hipStreamEndCapture(idx->stream, &g);
+                if (r) {
+                    if (g) hipGraphDestroy(g);
+                    return r;
+                }
+                if (ec != hipSuccess) return fail(GALN_HIP_E_DEVICE, "graph capture: %s", hipGetErrorString(ec));
+                const hipError_t ei = hipGraphInstantiate(&rs->gexec, g, nullptr, nullptr, 0);
+                hipGraphDestroy(g);
+                if (ei != hipSuccess) {
+                    rs->gexec = nullptr;
+                    return fail(GALN_HIP_E_DEVICE, "graph instantiate: %s", hipGetErrorString(ei));
+                }
+                rs->gkey = key;
+                rs->glaunches = rs->launches;
+            }
+            rs->launches = rs->glaunches;
+            HIP_OK(hipGraphLaunch(rs->gexec, idx->stream));
+        } else {
+            r = run_tier(rs, flags, t0, list, 0, rs->n, true, slots, 0, block, 0, budget);
+            int32_t nh = 0;
+            if (!r && budget && !(r = read_ctr(rs, 3, &nh)) && nh > 0)
+                r = run_heavy(rs, flags, rs->d_heavy, nh, 256, 0, 10, hwaves);
+        }
     }
     if (r) return r;
     int32_t nf = 0;
@@ -2639,6 +2765,7 @@ int galn_hip_readset_free(galn_hip_readset *rs)
     for (auto &e : rs->ev)
         if (e) hipEventDestroy(e);
     for (auto &e : rs->lev) hipEventDestroy(e);
+    if (rs->gexec) hipGraphExecDestroy(rs->gexec);
     rs->idx->n_readsets.fetch_sub(1);
     delete rs;
     return 0;

@@ -20,6 +20,7 @@ GALN_BIN = os.path.join(PKG, "bin", "galn")
 GALN_HIP_TRACE = 0x1
 GALN_HIP_NOFUSE = 0x2
 GALN_HIP_STATS = 0x4
+GALN_HIP_GRAPH = 0x8
 FORM_PLAIN, FORM_BATCHED = 0, 1
 E_RECCAP = -5
 

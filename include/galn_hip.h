@@ -48,6 +48,8 @@ extern "C" {
 #define GALN_HIP_TRACE   0x1u        /* compute the per-read trace hash (pop-order check) */
 #define GALN_HIP_NOFUSE  0x2u        /* never keep the exact child in registers (testing) */
 #define GALN_HIP_STATS   0x4u        /* count the wave-per-read engine's rounds and clocks */
+#define GALN_HIP_GRAPH   0x8u        /* batched form: run the launches as a captured hipGraph (replayed
+                                        while the read set, tuning and buffers are unchanged) */
 
 /* Index descriptor: field order of bwtgpu_meta_t, src/bwtsearch.h:26-31. */
 typedef struct {

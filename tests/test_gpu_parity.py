@@ -113,6 +113,28 @@ def test_autotuned_search_gives_the_golden_sai(golden_dir):
     assert fx.md5_bytes(nv.sai_bytes(o, res.n_aln, res.rec_off, res.rec)) == w["meta"]["runs"]["adna_1"]["sai_md5"]
 
 
+@pytest.mark.parametrize("name,key", [("adna_mixed", "adna_1"), ("serving_mix", "default_1")])
+def test_graph_replays_equal_eager_and_golden(name, key, golden_dir):
+    """The batched form run as a captured graph: every replay, with eager searches between them,
+    gives the eager result and the reference's .sai, pops and traces."""
+    print(f"case graph {name}/{key}", flush=True)
+    w, o, rs = readset_for(name, key, golden_dir)
+    exp = w["expected"]
+    rs.search(nv.FORM_BATCHED, nv.GALN_HIP_TRACE)
+    eager = rs.results(want_pops=True, want_trace=True)
+    for rep in range(3):
+        rs.search(nv.FORM_BATCHED, nv.GALN_HIP_TRACE | nv.GALN_HIP_GRAPH)
+        g = rs.results(want_pops=True, want_trace=True)
+        np.testing.assert_array_equal(g.n_aln, eager.n_aln)
+        np.testing.assert_array_equal(g.rec, eager.rec)
+        np.testing.assert_array_equal(g.pops, exp[key + "_pops"])
+        np.testing.assert_array_equal(g.trace, exp[key + "_trace"])
+        assert fx.md5_bytes(nv.sai_bytes(o, g.n_aln, g.rec_off, g.rec)) == w["meta"]["runs"][key]["sai_md5"]
+        assert [l["tier"] for l in rs.launches()][:2] == [0, 10]
+        rs.search(nv.FORM_PLAIN)  # eager work between replays
+        rs.results()
+
+
 def test_repeated_searches_identical(golden_dir):
     w, o, rs = readset_for("human_like", "adna_1", golden_dir)
     outs = []
