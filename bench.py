@@ -104,16 +104,22 @@ def cpu_baseline(w, res, sample: int, threads: int, workdir: str) -> dict:
 TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "k_search_k_heavy_adna10M_traffic.md")
 
 
-def measured_traffic(kernel: str, args) -> int | None:
+def measured_traffic(kernel: str, args, tuned: dict) -> int | None:
     """HBM bytes per launch of `kernel` measured by the FETCH_SIZE / WRITE_SIZE passes recorded
-    in profiles/ (scaled by a device copy of known size), for the default workload only."""
+    in profiles/ (scaled by a device copy of known size): for the default workload run with the
+    tuning that was profiled, else None."""
     if args.reads != 10_000_000 or args.ref_bp != 100_000_000 or not os.path.exists(TRAFFIC_PROFILE):
         return None
+    value, config = None, None
     for ln in open(TRAFFIC_PROFILE):
         f = ln.split()
         if len(f) == 3 and f[0] == "traffic_bytes_per_launch" and f[1] == kernel:
-            return int(f[2])
-    return None
+            value = int(f[2])
+        if len(f) == 5 and f[0] == "traffic_config":
+            config = {f[1]: int(f[2]), f[3]: int(f[4])}
+    if config is None or any(tuned.get(k) != v for k, v in config.items()):
+        return None
+    return value
 
 
 def serving_sweep(w, index, nv, loads, seconds: float, max_req: int, seed: int = 7) -> dict:
@@ -312,7 +318,7 @@ def main() -> int:
     k_bytes = first[0]["algo_bytes"]
     achieved = k_bytes / (k_ms * 1e-3) / 1e9
     total_algo = int(res_b.occ_bytes.sum())
-    traffic = measured_traffic(dom, args)
+    traffic = measured_traffic(dom, args, tuned)
     # per-read backtracking (popped stack entries), log2 bins: [0], [1], [2,3], [4,7], ...
     pops = res_b.pops.astype(np.int64)
     bins = np.bincount(np.where(pops > 0, np.floor(np.log2(np.maximum(pops, 1))).astype(np.int64) + 1, 0))

@@ -1998,7 +1998,8 @@ struct galn_hip_readset {
     };
     std::vector<Launch> launches;
     // the batched form's captured graph (GALN_HIP_GRAPH), its key and its launch records
-    hipGraphExec_t gexec = nullptr;
+    hipGraphExec_t gexec = nullptr, gexec2 = nullptr;  // prologue + first tier; wave engine
+    bool capturing = false;  // launches skip their timing events (recorded around the graphs)
     GraphKey gkey{};
     std::vector<Launch> glaunches;
     std::vector<hipEvent_t> lev;           // 2 per launch, reused across searches
@@ -2289,10 +2290,10 @@ int run_tier(galn_hip_readset *rs, uint32_t flags, const Tier &t, const int32_t 
     a.algo_bytes = rs->d_algo + li;
     rs->launches.push_back({tier_id, li, 0.0, 0});
     const int grid = (slots + block - 1) / block;
-    HIP_OK(hipEventRecord(rs->lev[2 * li], idx->stream));
+    if (!rs->capturing) HIP_OK(hipEventRecord(rs->lev[2 * li], idx->stream));
     r = launch(idx, a, block, grid, persist, (flags & GALN_HIP_TRACE) != 0, buckets_of(rs));
     if (r) return r;
-    HIP_OK(hipEventRecord(rs->lev[2 * li + 1], idx->stream));
+    if (!rs->capturing) HIP_OK(hipEventRecord(rs->lev[2 * li + 1], idx->stream));
     return 0;
 }
 
@@ -2368,13 +2369,13 @@ int run_heavy(galn_hip_readset *rs, uint32_t flags, const int32_t *list, int n, 
     }
     a.algo_bytes = rs->d_algo + li;
     rs->launches.push_back({tier_id, li, 0.0, 0});
-    HIP_OK(hipEventRecord(rs->lev[2 * li], idx->stream));
+    if (!rs->capturing) HIP_OK(hipEventRecord(rs->lev[2 * li], idx->stream));
     if (flags & GALN_HIP_TRACE)
         hipLaunchKernelGGL(k_heavy<true>, dim3(waves), dim3(HW), lds, idx->stream, a);
     else
         hipLaunchKernelGGL(k_heavy<false>, dim3(waves), dim3(HW), lds, idx->stream, a);
     HIP_OK(hipGetLastError());
-    HIP_OK(hipEventRecord(rs->lev[2 * li + 1], idx->stream));
+    if (!rs->capturing) HIP_OK(hipEventRecord(rs->lev[2 * li + 1], idx->stream));
     return 0;
 }
 
@@ -2513,30 +2514,51 @@ int readset_search(galn_hip_readset *rs, int form, uint32_t flags, const galn_hi
             }
             fill_key_ptrs(rs, key);
             if (!rs->gexec || memcmp(&key, &rs->gkey, sizeof key) != 0) {
-                if (rs->gexec) HIP_OK(hipGraphExecDestroy(rs->gexec));
-                rs->gexec = nullptr;
-                HIP_OK(hipStreamBeginCapture(idx->stream, hipStreamCaptureModeThreadLocal));
-                r = prologue();
-                if (!r) r = run_tier(rs, flags, t0, list, 0, rs->n, true, slots, 0, block, 0, budget);
-                if (!r && budget) r = run_heavy(rs, flags, rs->d_heavy, -1, 256, 0, 10, hwaves);
-                hipGraph_t g = nullptr;
-                const hipError_t ec = hipStreamEndCapture(idx->stream, &g);
-                if (r) {
-                    if (g) hipGraphDestroy(g);
-                    return r;
+                for (hipGraphExec_t *ge : {&rs->gexec, &rs->gexec2}) {
+                    if (*ge) HIP_OK(hipGraphExecDestroy(*ge));
+                    *ge = nullptr;
                 }
-                if (ec != hipSuccess) return fail(GALN_HIP_E_DEVICE, "graph capture: %s", hipGetErrorString(ec));
-                const hipError_t ei = hipGraphInstantiate(&rs->gexec, g, nullptr, nullptr, 0);
-                hipGraphDestroy(g);
-                if (ei != hipSuccess) {
-                    rs->gexec = nullptr;
-                    return fail(GALN_HIP_E_DEVICE, "graph instantiate: %s", hipGetErrorString(ei));
-                }
+                // two graphs, so that the timing events sit between them rather than inside
+                auto capture = [&](int part, hipGraphExec_t *out) -> int {
+                    HIP_OK(hipStreamBeginCapture(idx->stream, hipStreamCaptureModeThreadLocal));
+                    rs->capturing = true;
+                    int rc = 0;
+                    if (part == 0) {
+                        rc = prologue();
+                        if (!rc) rc = run_tier(rs, flags, t0, list, 0, rs->n, true, slots, 0, block, 0, budget);
+                    } else {
+                        rc = run_heavy(rs, flags, rs->d_heavy, -1, 256, 0, 10, hwaves);
+                    }
+                    rs->capturing = false;
+                    hipGraph_t g = nullptr;
+                    const hipError_t ec = hipStreamEndCapture(idx->stream, &g);
+                    if (rc) {
+                        if (g) hipGraphDestroy(g);
+                        return rc;
+                    }
+                    if (ec != hipSuccess) return fail(GALN_HIP_E_DEVICE, "graph capture: %s", hipGetErrorString(ec));
+                    const hipError_t ei = hipGraphInstantiate(out, g, nullptr, nullptr, 0);
+                    hipGraphDestroy(g);
+                    if (ei != hipSuccess) {
+                        *out = nullptr;
+                        return fail(GALN_HIP_E_DEVICE, "graph instantiate: %s", hipGetErrorString(ei));
+                    }
+                    return 0;
+                };
+                if ((r = capture(0, &rs->gexec))) return r;
+                if (budget && (r = capture(1, &rs->gexec2))) return r;
                 rs->gkey = key;
                 rs->glaunches = rs->launches;
             }
             rs->launches = rs->glaunches;
+            HIP_OK(hipEventRecord(rs->lev[0], idx->stream));
             HIP_OK(hipGraphLaunch(rs->gexec, idx->stream));
+            HIP_OK(hipEventRecord(rs->lev[1], idx->stream));
+            if (rs->gexec2) {
+                HIP_OK(hipEventRecord(rs->lev[2], idx->stream));
+                HIP_OK(hipGraphLaunch(rs->gexec2, idx->stream));
+                HIP_OK(hipEventRecord(rs->lev[3], idx->stream));
+            }
         } else {
             r = run_tier(rs, flags, t0, list, 0, rs->n, true, slots, 0, block, 0, budget);
             int32_t nh = 0;
@@ -2766,6 +2788,7 @@ int galn_hip_readset_free(galn_hip_readset *rs)
         if (e) hipEventDestroy(e);
     for (auto &e : rs->lev) hipEventDestroy(e);
     if (rs->gexec) hipGraphExecDestroy(rs->gexec);
+    if (rs->gexec2) hipGraphExecDestroy(rs->gexec2);
     rs->idx->n_readsets.fetch_sub(1);
     delete rs;
     return 0;
