@@ -31,6 +31,9 @@
 // the capacities chosen (the reference's contract, src/bwtgap_hd.h:10-22).
 
 #include <hip/hip_runtime.h>
+#include <cstring>
+#include <rocprim/device/device_scan.hpp>
+#include <rocprim/iterator/transform_iterator.hpp>
 
 #include <algorithm>
 #include <chrono>
@@ -1886,6 +1889,24 @@ __global__ void __launch_bounds__(64) k_heavy(HArgs a)
     if (a.stats && lane < 16) atomicAdd(a.stats + lane, F.st[lane]);
 }
 
+// A read's record count for the output offsets (error codes count none).
+struct RecCount {
+    __host__ __device__ uint64_t operator()(int32_t v) const { return v > 0 ? (uint64_t)v : 0; }
+};
+
+// Records of every read in read order: read i's n_aln[i] records move from the arena (at
+// arena_off[i], where the search appended them) to incl[i] - n_aln[i] of the staging array.
+__global__ void k_gather_records(const int32_t *n_aln, const uint64_t *arena_off, const uint64_t *incl,
+                                 const galn_aln_rec_t *arena, galn_aln_rec_t *stage, uint64_t *rec_off, int n)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int na = n_aln[i] > 0 ? n_aln[i] : 0;
+    const uint64_t o = incl[i] - (uint64_t)na;
+    rec_off[i] = o;
+    for (int j = 0; j < na; ++j) stage[o + j] = arena[arena_off[i] + j];
+}
+
 // Fills n words with v: the search's counter and pool-head resets, as kernels so that a
 // captured graph holds no memset nodes.
 __global__ void k_fill_u32(uint32_t *p, uint32_t v, uint32_t n)
@@ -1980,6 +2001,12 @@ struct galn_hip_readset {
     size_t scratch_bytes = 0;
     // host copies
     std::vector<galn_search_opt_t> h_bopts;
+    // results in read order: inclusive prefix sums of record counts, output offsets, staging
+    uint64_t *d_incl = nullptr, *d_roff = nullptr;
+    galn_aln_rec_t *d_stage = nullptr;
+    uint64_t stage_cap = 0;
+    void *d_scan_tmp = nullptr;
+    size_t scan_tmp_bytes = 0;
     // capacities of the device arrays (reads, query bytes, batches)
     int cap_n = 0, cap_b = 0;
     uint64_t cap_q = 0;
@@ -2491,6 +2518,9 @@ int readset_search(galn_hip_readset *rs, int form, uint32_t flags, const galn_hi
     } else {
         // batched: one persistent launch over the reads, longest first; reads that spend their
         // pop budget go to the wave-per-read engine
+        // 64-thread workgroups unless tuned otherwise (the autotuner's choice on every workload
+        // measured so far)
+        block = 64;
         if (tune && tune->block_threads > 0 && buckets_of(rs) * tune->block_threads * 4 <= 65536)
             block = tune->block_threads;
         slots = resident_slots(rs, block, trace, tune ? tune->waves_per_cu : 0);
@@ -2630,7 +2660,8 @@ int readset_reserve(galn_hip_readset *rs, int n, uint64_t qbytes, int nb)
              regrow((void **)&rs->d_out_off, c * 8) && regrow((void **)&rs->d_pops, c * 8) &&
              regrow((void **)&rs->d_trace, c * 8) && regrow((void **)&rs->d_bytes, c * 8) &&
              regrow((void **)&rs->d_fail[0], c * 4) && regrow((void **)&rs->d_fail[1], c * 4) &&
-             regrow((void **)&rs->d_heavy, c * 4);
+             regrow((void **)&rs->d_heavy, c * 4) && regrow((void **)&rs->d_incl, c * 8) &&
+             regrow((void **)&rs->d_roff, c * 8);
         rs->cap_n = ok ? (int)c : 0;
         if (ok && rs->rec_cap < 2 * c) {
             rs->rec_cap = std::max<uint64_t>(2 * c, 1 << 16);
@@ -2781,7 +2812,7 @@ int galn_hip_readset_free(galn_hip_readset *rs)
     void *ptrs[] = {rs->d_query, rs->d_qoff, rs->d_qlen, rs->d_maxdiff, rs->d_batch, rs->d_nstacks, rs->d_bopts,
                     rs->d_sorted, rs->d_out_n, rs->d_out_off, rs->d_pops, rs->d_trace, rs->d_bytes, rs->d_rec, rs->d_top,
                     rs->d_fail[0], rs->d_fail[1], rs->d_ctr, rs->d_scratch, rs->d_algo, rs->d_heavy, rs->d_hscratch,
-                    rs->d_stats};
+                    rs->d_stats, rs->d_incl, rs->d_roff, rs->d_stage, rs->d_scan_tmp};
     for (void *p : ptrs)
         if (p) hipFree(p);
     for (auto &e : rs->ev)
@@ -2897,36 +2928,43 @@ int galn_hip_readset_results(galn_hip_readset *rs, galn_hip_out *out)
         return 0;
     }
     hipStream_t s = idx->stream;
-    std::vector<uint64_t> off(n);
-    unsigned long long top = 0;
+    // records in read order, arranged on the device: offsets = prefix sums of the record counts
+    // (errors count 0), then every read's records gathered from the arena
+    size_t tmp_bytes = 0;
+    const rocprim::transform_iterator<const int32_t *, RecCount, uint64_t> cnt(rs->d_out_n, RecCount());
+    HIP_OK(rocprim::inclusive_scan(nullptr, tmp_bytes, cnt, rs->d_incl, (size_t)n, rocprim::plus<uint64_t>(), s));
+    if (tmp_bytes > rs->scan_tmp_bytes) {
+        if (rs->d_scan_tmp) HIP_OK(hipFree(rs->d_scan_tmp));
+        rs->d_scan_tmp = nullptr;
+        HIP_OK(hipMalloc(&rs->d_scan_tmp, tmp_bytes));
+        rs->scan_tmp_bytes = tmp_bytes;
+    }
+    HIP_OK(rocprim::inclusive_scan(rs->d_scan_tmp, tmp_bytes, cnt, rs->d_incl, (size_t)n, rocprim::plus<uint64_t>(), s));
+    unsigned long long need = 0;
+    HIP_OK(hipMemcpyAsync(&need, rs->d_incl + (n - 1), 8, hipMemcpyDeviceToHost, s));
     HIP_OK(hipMemcpyAsync(out->n_aln, rs->d_out_n, (size_t)n * 4, hipMemcpyDeviceToHost, s));
-    HIP_OK(hipMemcpyAsync(off.data(), rs->d_out_off, (size_t)n * 8, hipMemcpyDeviceToHost, s));
-    HIP_OK(hipMemcpyAsync(&top, rs->d_top, 8, hipMemcpyDeviceToHost, s));
     if (out->n_pops) HIP_OK(hipMemcpyAsync(out->n_pops, rs->d_pops, (size_t)n * 8, hipMemcpyDeviceToHost, s));
     if (out->trace) HIP_OK(hipMemcpyAsync(out->trace, rs->d_trace, (size_t)n * 8, hipMemcpyDeviceToHost, s));
     if (out->occ_bytes) HIP_OK(hipMemcpyAsync(out->occ_bytes, rs->d_bytes, (size_t)n * 8, hipMemcpyDeviceToHost, s));
     HIP_OK(hipStreamSynchronize(s));
-    uint64_t need = 0;
-    for (int i = 0; i < n; ++i)
-        if (out->n_aln[i] > 0) need += (uint64_t)out->n_aln[i];
     out->n_rec = need;
     if (need > out->rec_cap || (need > 0 && !out->rec))
         return fail(GALN_HIP_E_RECCAP, "record array holds %llu, %llu needed", (unsigned long long)out->rec_cap,
                     (unsigned long long)need);
-    const uint64_t used = std::min<uint64_t>(top, rs->rec_cap);
-    std::vector<galn_aln_rec_t> arena(used);
-    if (used) HIP_OK(hipMemcpyAsync(arena.data(), rs->d_rec, used * sizeof(galn_aln_rec_t), hipMemcpyDeviceToHost, s));
-    HIP_OK(hipStreamSynchronize(s));
-    // records in read order
-    uint64_t o = 0;
-    for (int i = 0; i < n; ++i) {
-        out->rec_off[i] = o;
-        const int na = out->n_aln[i];
-        if (na > 0) {
-            memcpy(out->rec + o, arena.data() + off[i], (size_t)na * sizeof(galn_aln_rec_t));
-            o += (uint64_t)na;
-        }
+    if (rs->stage_cap < need) {
+        if (rs->d_stage) HIP_OK(hipFree(rs->d_stage));
+        rs->d_stage = nullptr;
+        rs->stage_cap = 0;
+        const uint64_t cap = std::max<uint64_t>(need, rs->rec_cap);
+        HIP_OK(hipMalloc(&rs->d_stage, cap * sizeof(galn_aln_rec_t)));
+        rs->stage_cap = cap;
     }
+    hipLaunchKernelGGL(k_gather_records, dim3((n + 255) / 256), dim3(256), 0, s, rs->d_out_n, rs->d_out_off,
+                       (const uint64_t *)rs->d_incl, (const galn_aln_rec_t *)rs->d_rec, rs->d_stage, rs->d_roff, n);
+    HIP_OK(hipGetLastError());
+    HIP_OK(hipMemcpyAsync(out->rec_off, rs->d_roff, (size_t)n * 8, hipMemcpyDeviceToHost, s));
+    if (need) HIP_OK(hipMemcpyAsync(out->rec, rs->d_stage, need * sizeof(galn_aln_rec_t), hipMemcpyDeviceToHost, s));
+    HIP_OK(hipStreamSynchronize(s));
     return 0;
 }
 

@@ -410,14 +410,15 @@ class ReadSet:
 
     def results(self, want_pops=True, want_trace=False, want_bytes=False, rec_cap: int | None = None) -> Results:
         n = self.n
-        n_aln = np.zeros(max(n, 1), np.int32)
-        rec_off = np.zeros(max(n, 1), np.uint64)
-        pops = np.zeros(max(n, 1), np.uint64) if want_pops else None
-        trace = np.zeros(max(n, 1), np.uint64) if want_trace else None
-        ob = np.zeros(max(n, 1), np.uint64) if want_bytes else None
+        # every element is written by the library
+        n_aln = np.empty(max(n, 1), np.int32)
+        rec_off = np.empty(max(n, 1), np.uint64)
+        pops = np.empty(max(n, 1), np.uint64) if want_pops else None
+        trace = np.empty(max(n, 1), np.uint64) if want_trace else None
+        ob = np.empty(max(n, 1), np.uint64) if want_bytes else None
         cap = rec_cap if rec_cap is not None else max(2 * n, 1)
         while True:
-            rec = np.zeros((max(cap, 1), 3), np.uint64)
+            rec = np.empty((max(cap, 1), 3), np.uint64)
             O = Out()
             O.n_aln, O.rec_off, O.rec, O.rec_cap = _p(n_aln), _p(rec_off), _p(rec), cap
             O.n_pops = _p(pops) if pops is not None else None

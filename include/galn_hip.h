@@ -117,7 +117,7 @@ typedef struct {
 
 /* Tuning of the batched form (0 = library default / autotuned). */
 typedef struct {
-    int32_t block_threads;     /* threads per workgroup (64 or 128) */
+    int32_t block_threads;     /* threads per workgroup (64 or 128; default 64) */
     int32_t waves_per_cu;      /* persistent grid size = waves_per_cu * CUs */
     int32_t chunks_per_lane;   /* stack chunks (16 entries each) per lane in the first tier */
     int32_t rec_per_lane;      /* record slots per lane in the first tier */
