@@ -136,6 +136,11 @@ struct KArgs {
     uint64_t pop_budget;     // > 0: reads popping more entries go to heavy_list
     int32_t *heavy_list;
     int32_t *heavy_n;
+    // hand-off of a heavy read's widths: slot h (its place in heavy_list) holds
+    // [bytes | 1 << 63][W[0..len]][SW[0..seed_len]], or 0 when the widths were not kept
+    uint64_t *hand;
+    uint32_t hand_stride;    // u64 words per slot
+    int32_t hand_cap;        // slots
 };
 
 // The kernel argument segment as k_search reads it, and a load that the compiler must perform
@@ -344,7 +349,7 @@ __global__ void __launch_bounds__(128) k_search(KArgs a)
     uint32_t spare = NIL, free_head = NIL;
     int n_own = 0, own_next = 0, gleft = 0;  // granules held, next one to use, chunks left in it
     uint32_t gbase = 0;
-    uint64_t pops = 0, trace = TR_INIT, obytes = 0;
+    uint64_t pops = 0, trace = TR_INIT, obytes = 0, wbytes = 0;  // wbytes: of the widths
     // what the next iteration loads, and what it then does
     int phase = PH_CLAIM;
     bool rq_lf = false, rq_ent = false, rq_pf = false, rq_w = false;
@@ -482,7 +487,21 @@ __global__ void __launch_bounds__(128) k_search(KArgs a)
     auto finish = [&](int code) __attribute__((always_inline)) {
         int n = code;
         if (n == ERR_HEAVY) {
-            CA(heavy_list)[atomicAdd(CA(heavy_n), 1)] = rid;
+            const int h = atomicAdd(CA(heavy_n), 1);
+            CA(heavy_list)[h] = rid;
+            if (h < CA(hand_cap)) {
+                // the widths go along while no hit has changed them (gap_shadow only runs with a
+                // new record); the wave engine then skips computing them again
+                uint64_t *const dst = CA(hand) + (uint64_t)h * CA(hand_stride);
+                if (n_aln == 0) {
+                    for (int j = 0; j <= len; ++j) dst[1 + j] = W[j];
+                    if (seeding)
+                        for (int j = 0; j <= seed_len; ++j) dst[2 + len + j] = SW[j];
+                    dst[0] = wbytes | (1ull << 63);
+                } else {
+                    dst[0] = 0;
+                }
+            }
         } else {
             if (n > 0) {
                 const unsigned long long b0 = atomicAdd(CA(out_top), (unsigned long long)n);
@@ -685,6 +704,7 @@ __global__ void __launch_bounds__(128) k_search(KArgs a)
     };
 
     auto start_search = [&]() __attribute__((always_inline)) -> int {
+        wbytes = obytes;
         best_score = score_of(md_read + 1, max_gapo + 1, max_gape + 1);
         best_diff = md_read + 1;
         max_diff = md_read;
@@ -1103,6 +1123,9 @@ struct HArgs {
     const int32_t *list;
     int32_t n_list;
     const int32_t *n_list_dev;  // when set, the number of reads is read here (written by k_search)
+    const uint64_t *hand;       // widths handed over by k_search (slot = list position), or null
+    uint32_t hand_stride;
+    int32_t hand_cap;
     int32_t *qnext;
     int32_t n_slots;
     uint8_t *scratch;
@@ -1283,7 +1306,18 @@ __global__ void __launch_bounds__(64) k_heavy(HArgs a)
         __syncthreads();
         const bool unsup = len < 1 || len > MAXLEN || len + 1 > a.wlen || F.n_stacks > NB || F.md_read > MAXDIFF ||
                            F.max_gapo > MAXDIFF || F.max_gapo + F.max_gape > 255;
-        if (!unsup) {
+        const uint64_t h0 = (!unsup && a.hand && pos < a.hand_cap) ? a.hand[(uint64_t)pos * a.hand_stride] : 0;
+        if (!unsup && (h0 >> 63)) {
+            // the widths k_search computed for this read
+            const uint64_t *src = a.hand + (uint64_t)pos * a.hand_stride;
+            for (int j = lane; j <= len; j += HW) W[j] = src[1 + j];
+            if (F.seeding)
+                for (int j = lane; j <= F.seed_len; j += HW) SW[j] = src[2 + len + j];
+            if (lane == 0) F.obytes = h0 & ~(1ull << 63);
+            int nN = 0;
+            for (int j = lane; j < len; j += HW) nN += q[j] > 3;
+            if (nN) atomicAdd(&F.nN, nN);
+        } else if (!unsup) {
             uint64_t wb = 0;
             if (lane == 0) wb = cal_width_one(a.bwt, M, len, q, W);
             if (lane == 1 && F.seeding) wb = cal_width_one(a.bwt, M, F.seed_len, q + (len - F.seed_len), SW);
@@ -1967,6 +2001,9 @@ struct GraphKey {
     const void *ptr[20];
     uint64_t rec_cap_out, n_gran, scratch_bytes, hscratch_bytes;
     int32_t max_len, seed_len, max_stacks, max_entries;
+    const void *hand;
+    int32_t hand_cap;
+    uint32_t hand_stride;
 };
 
 struct galn_hip_readset {
@@ -2001,6 +2038,10 @@ struct galn_hip_readset {
     size_t scratch_bytes = 0;
     // host copies
     std::vector<galn_search_opt_t> h_bopts;
+    // widths handed from k_search to k_heavy, one slot per heavy-list position
+    uint64_t *d_hand = nullptr;
+    uint32_t hand_stride = 0;
+    int32_t hand_cap = 0;
     // results in read order: inclusive prefix sums of record counts, output offsets, staging
     uint64_t *d_incl = nullptr, *d_roff = nullptr;
     galn_aln_rec_t *d_stage = nullptr;
@@ -2255,6 +2296,29 @@ int prepare_heavy(galn_hip_readset *rs, int waves, int rec_cap)
     return 0;
 }
 
+// Hand-off slots for the widths of reads moving to the wave engine: one per read, up to
+// HAND_BYTES (reads beyond it recompute their widths there).
+constexpr uint64_t HAND_BYTES = 4ull << 30;
+
+int prepare_hand(galn_hip_readset *rs)
+{
+    const int sl = (rs->seed_len > 0 && rs->seed_len < rs->max_len) ? rs->seed_len : 0;
+    const uint32_t stride = (uint32_t)(1 + (rs->max_len + 1) + (sl + 1));
+    const int cap = (int)std::min<uint64_t>((uint64_t)rs->n, HAND_BYTES / (stride * 8ull));
+    if (rs->d_hand && rs->hand_stride == stride && rs->hand_cap >= cap) return 0;
+    if (rs->d_hand) HIP_OK(hipFree(rs->d_hand));
+    rs->d_hand = nullptr;
+    rs->hand_cap = 0;
+    if (hipMalloc(&rs->d_hand, (size_t)std::max(cap, 1) * stride * 8) != hipSuccess) {
+        rs->d_hand = nullptr;  // no hand-off: the wave engine computes the widths itself
+        (void)hipGetLastError();
+        return 0;
+    }
+    rs->hand_stride = stride;
+    rs->hand_cap = cap;
+    return 0;
+}
+
 // What a captured search graph holds: when any of it changes, the graph is captured again.
 void fill_key_ptrs(const galn_hip_readset *rs, GraphKey &k)
 {
@@ -2269,6 +2333,7 @@ void fill_key_ptrs(const galn_hip_readset *rs, GraphKey &k)
     k.scratch_bytes = rs->scratch_bytes;
     k.hscratch_bytes = rs->hscratch_bytes;
     k.max_len = rs->max_len, k.seed_len = rs->seed_len, k.max_stacks = rs->max_stacks, k.max_entries = rs->max_entries;
+    k.hand = rs->d_hand, k.hand_cap = rs->hand_cap, k.hand_stride = rs->hand_stride;
 }
 
 int run_tier(galn_hip_readset *rs, uint32_t flags, const Tier &t, const int32_t *list, int list_base, int n_list,
@@ -2303,6 +2368,9 @@ int run_tier(galn_hip_readset *rs, uint32_t flags, const Tier &t, const int32_t 
     a.pop_budget = pop_budget;
     a.heavy_list = rs->d_heavy;
     a.heavy_n = rs->d_ctr + 3;
+    a.hand = pop_budget ? rs->d_hand : nullptr;
+    a.hand_stride = rs->hand_stride;
+    a.hand_cap = (pop_budget && rs->d_hand) ? rs->hand_cap : 0;
     if (persist) FILL(rs->d_ctr + 2, 0, sizeof(int32_t));
     // an empty free stack (tag and granule all ones) and no granule handed out yet
     FILL(idx->d_ghead, 0xffffffffu, NSHARD * 8);
@@ -2354,6 +2422,9 @@ int run_heavy(galn_hip_readset *rs, uint32_t flags, const int32_t *list, int n, 
     a.list = list;
     a.n_list = dev_count ? 0 : n;
     a.n_list_dev = dev_count ? rs->d_ctr + 3 : nullptr;
+    a.hand = rs->d_hand;  // the list is the heavy list of the k_search launch just before
+    a.hand_stride = rs->hand_stride;
+    a.hand_cap = rs->d_hand ? rs->hand_cap : 0;
     a.qnext = rs->d_ctr + 2;
     a.n_slots = waves;
     a.scratch = rs->d_hscratch;
@@ -2528,6 +2599,7 @@ int readset_search(galn_hip_readset *rs, int form, uint32_t flags, const galn_hi
         const int hp = tune ? tune->heavy_pops : 0;
         budget = (hp < 0 || !heavy_ok(rs)) ? 0 : (uint64_t)(hp > 0 ? hp : DEFAULT_HEAVY_POPS);
         hwaves = heavy_waves(rs, flags);
+        if (budget && (r = prepare_hand(rs))) return r;
         const int32_t *list = sorted ? rs->d_sorted : nullptr;
         if (graph) {
             // the prologue, the persistent launch and the wave-engine launch (its read count read
@@ -2812,7 +2884,7 @@ int galn_hip_readset_free(galn_hip_readset *rs)
     void *ptrs[] = {rs->d_query, rs->d_qoff, rs->d_qlen, rs->d_maxdiff, rs->d_batch, rs->d_nstacks, rs->d_bopts,
                     rs->d_sorted, rs->d_out_n, rs->d_out_off, rs->d_pops, rs->d_trace, rs->d_bytes, rs->d_rec, rs->d_top,
                     rs->d_fail[0], rs->d_fail[1], rs->d_ctr, rs->d_scratch, rs->d_algo, rs->d_heavy, rs->d_hscratch,
-                    rs->d_stats, rs->d_incl, rs->d_roff, rs->d_stage, rs->d_scan_tmp};
+                    rs->d_stats, rs->d_incl, rs->d_roff, rs->d_stage, rs->d_scan_tmp, rs->d_hand};
     for (void *p : ptrs)
         if (p) hipFree(p);
     for (auto &e : rs->ev)
