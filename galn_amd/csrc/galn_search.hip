@@ -1227,7 +1227,7 @@ __device__ uint64_t cal_width_one(const uint32_t *bwt, const DMeta &M, int len, 
 }
 
 template <bool TRACE>
-__global__ void __launch_bounds__(64) k_heavy(HArgs a)
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) k_heavy(HArgs a)
 {
     extern __shared__ __align__(16) uint8_t hlds[];
     HShared &F = *reinterpret_cast<HShared *>(hlds);
@@ -1620,6 +1620,11 @@ __global__ void __launch_bounds__(64) k_heavy(HArgs a)
                 by += SINFO[t] >> 8;
                 ncp += SINFO[t] & 0xff;
             }
+            // the first committed pushes are loaded now, to arrive while the plan is applied
+            Ent pu[4];
+#pragma unroll
+            for (int t = 0; t < 4; ++t)
+                if (t < ncp) pu[t] = PLOG[t];
             const unsigned long long sum_pops = wave_sum_u64(cj), sum_by = wave_sum_u64(by);
             const int sum_ne = wave_sum_i32(cj > 0 ? ncp - 1 : 0);
             if (TRACE) {
@@ -1669,20 +1674,27 @@ __global__ void __launch_bounds__(64) k_heavy(HArgs a)
             // (a mismatch, a gap open, a gap extension); its rank in the bucket is its place in
             // the sequential push order, chain by chain ----
             const int tg0 = b + s_mm, tg1 = b + s_gapo, tg2 = b + s_gape;
-            int n0 = 0, n1 = 0, n2 = 0, bad = 0;
-            for (int u = 0; u < ncp; ++u) {
-                int x;
-                if (u < 32) {
-                    x = (int)(pcode >> (2 * u)) & 3;
-                } else {
-                    const Ent pu = PLOG[u];
-                    const int sc = e_mm(pu) * s_mm + e_go(pu) * s_gapo + e_ge(pu) * s_gape;  // its bucket
-                    x = sc == tg0 ? 0 : sc == tg1 ? 1 : sc == tg2 ? 2 : 3;
+            // pushes per bucket: the first 32 from their 2-bit codes, the rest from the log
+            const uint64_t ODD = 0x5555555555555555ull;
+            const uint64_t vmask = ncp >= 32 ? ODD : (((uint64_t)1 << (2 * ncp)) - 1) & ODD;
+            const uint64_t clo = pcode & vmask, chi = (pcode >> 1) & vmask;
+            int n0 = __builtin_popcountll(vmask & ~clo & ~chi), n1 = __builtin_popcountll(clo & ~chi);
+            int n2 = __builtin_popcountll(chi & ~clo), bad = (clo & chi) != 0;
+            for (int u0 = 32; u0 < ncp; u0 += 4) {
+                Ent pv[4];
+#pragma unroll
+                for (int t = 0; t < 4; ++t)
+                    if (u0 + t < ncp) pv[t] = PLOG[u0 + t];
+#pragma unroll
+                for (int t = 0; t < 4; ++t) {
+                    if (u0 + t >= ncp) break;
+                    const int sc = e_mm(pv[t]) * s_mm + e_go(pv[t]) * s_gapo + e_ge(pv[t]) * s_gape;  // its bucket
+                    const int x = sc == tg0 ? 0 : sc == tg1 ? 1 : sc == tg2 ? 2 : 3;
+                    n0 += x == 0;
+                    n1 += x == 1;
+                    n2 += x == 2;
+                    bad |= x == 3;
                 }
-                n0 += x == 0;
-                n1 += x == 1;
-                n2 += x == 2;
-                bad |= x == 3;
             }
             const int r0 = wave_excl_scan(n0), r1 = wave_excl_scan(n1), r2 = wave_excl_scan(n2);
             const int T0 = __shfl(r0 + n0, HW - 1), T1 = __shfl(r1 + n1, HW - 1), T2 = __shfl(r2 + n2, HW - 1);
@@ -1753,13 +1765,14 @@ __global__ void __launch_bounds__(64) k_heavy(HArgs a)
             const long long t_c2 = a.stats ? wall_clock64() : 0;
             {
                 uint32_t q0 = (uint32_t)r0, q1 = (uint32_t)r1, q2 = (uint32_t)r2;
-                const int tgs[3] = {tg0, tg1, tg2};
-                // four log entries loaded together, then placed
+                const int fp0 = first_piece[tg0], fp1 = first_piece[tg1], fp2 = first_piece[tg2];
+                // four log entries loaded together, then placed (the first four are loaded)
                 for (int u0 = 0; u0 < ncp; u0 += 4) {
-                    Ent pu[4];
+                    if (u0 > 0) {
 #pragma unroll
-                    for (int t = 0; t < 4; ++t)
-                        if (u0 + t < ncp) pu[t] = PLOG[u0 + t];
+                        for (int t = 0; t < 4; ++t)
+                            if (u0 + t < ncp) pu[t] = PLOG[u0 + t];
+                    }
 #pragma unroll
                     for (int t = 0; t < 4; ++t) {
                         const int u = u0 + t;
@@ -1772,7 +1785,7 @@ __global__ void __launch_bounds__(64) k_heavy(HArgs a)
                             x = sc == tg0 ? 0 : sc == tg1 ? 1 : 2;  // (3 was ruled out above)
                         }
                         const uint32_t r = x == 0 ? q0++ : x == 1 ? q1++ : q2++;
-                        int p = first_piece[tgs[x]];
+                        int p = x == 0 ? fp0 : x == 1 ? fp1 : fp2;
                         while (r >= pieces[p].y + pieces[p].z) ++p;
                         POOL[pieces[p].w + (r - pieces[p].y)] = pu[t];
                     }
