@@ -83,6 +83,18 @@ struct DMeta {
     uint64_t seq_len;
 };
 
+// The cumulative symbol counts held in registers: a lane's count for its own symbol is picked
+// by selects. Indexing the kernel-argument copy by a per-lane symbol is a vector memory load,
+// whose wait also waits for every store the wave still has in flight.
+struct L2Sel {
+    uint64_t v0, v1, v2, v3, v4;
+    __device__ explicit L2Sel(const DMeta &m) : v0(m.L2[0]), v1(m.L2[1]), v2(m.L2[2]), v3(m.L2[3]), v4(m.L2[4]) {}
+    __device__ __forceinline__ uint64_t operator[](int c) const
+    {
+        return c == 0 ? v0 : c == 1 ? v1 : c == 2 ? v2 : c == 3 ? v3 : v4;
+    }
+};
+
 // packed stack entry (layout below)
 struct __align__(16) Ent {
     uint64_t w0, w1;
@@ -1205,6 +1217,7 @@ __device__ __forceinline__ int wave_max_i32(int v)
 // bwt_cal_width into LDS by one lane (src/bwtgap_hd.h:216-242); returns the algorithmic bytes
 __device__ uint64_t cal_width_one(const uint32_t *bwt, const DMeta &M, int len, const uint8_t *str, uint64_t *w)
 {
+    const L2Sel L2s(M);
     uint64_t k = 0, l = M.seq_len, ca[4], cb[4], bytes = 0;
     int bid = 0;
     for (int i = 0; i < len; ++i) {
@@ -1212,8 +1225,8 @@ __device__ uint64_t cal_width_one(const uint32_t *bwt, const DMeta &M, int len, 
         const int c = c0 > 3 ? 4 : 3 - c0;
         if (c < 4) {
             bytes += lf_pair(bwt, M, k - 1, l, ca, cb);
-            k = M.L2[c] + ca[c] + 1;
-            l = M.L2[c] + cb[c];
+            k = L2s[c] + ca[c] + 1;
+            l = L2s[c] + cb[c];
         }
         if (k > l || c > 3) {
             k = 0;
@@ -1257,6 +1270,7 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) k_
     uint32_t *const SINFO = SINFO_ALL + lane * H_STEPS;
     Ent *const POOL = a.pool;
     const DMeta &M = a.meta;
+    const L2Sel L2s(M);
     const uint64_t seq_len = M.seq_len;
 
     if (lane == 0) {
@@ -1452,7 +1466,7 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) k_
                                 // exact tail: bwt_match_exact_alt, src/bwtgap_hd.h:195-211
                                 if (qbase <= 3) {
                                     sbytes += lfb;
-                                    uint64_t k = M.L2[qbase] + ca[qbase] + 1, l = M.L2[qbase] + cb[qbase];
+                                    uint64_t k = L2s[qbase] + ca[qbase] + 1, l = L2s[qbase] + cb[qbase];
                                     bool ok = k <= l;
                                     for (int jn = i0 - 2; ok && jn >= 0; --jn) {
                                         const int c = q[jn];
@@ -1461,8 +1475,8 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) k_
                                             break;
                                         }
                                         sbytes += lf_pair(a.bwt, M, k - 1, l, ca, cb);
-                                        k = M.L2[c] + ca[c] + 1;
-                                        l = M.L2[c] + cb[c];
+                                        k = L2s[c] + ca[c] + 1;
+                                        l = L2s[c] + cb[c];
                                         ok = k <= l;
                                     }
                                     if (ok && (int)(uint32_t)(l - k + 1) != 0) {
@@ -1516,7 +1530,7 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) k_
                                             const int sc = mm * s_mm + (go + 1) * s_gapo + ge * s_gape;
                                             logp(sc, mk_ent(k, l, i, i, mm, go + 1, ge, ins + 1, ST_I));
                                             for (int j = 0; j < 4; ++j) {
-                                                const uint64_t kk = M.L2[j] + ca[j] + 1, ll = M.L2[j] + cb[j];
+                                                const uint64_t kk = L2s[j] + ca[j] + 1, ll = L2s[j] + cb[j];
                                                 if (kk <= ll) logp(sc, mk_ent(kk, ll, i + 1, i + 1, mm, go + 1, ge, ins, ST_D));
                                             }
                                         }
@@ -1528,7 +1542,7 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) k_
                                         if (ge < max_gape && (ge + go < max_diff || occ < (uint64_t)(int64_t)mdo)) {
                                             const int sc = mm * s_mm + go * s_gapo + (ge + 1) * s_gape;
                                             for (int j = 0; j < 4; ++j) {
-                                                const uint64_t kk = M.L2[j] + ca[j] + 1, ll = M.L2[j] + cb[j];
+                                                const uint64_t kk = L2s[j] + ca[j] + 1, ll = L2s[j] + cb[j];
                                                 if (kk <= ll) logp(sc, mk_ent(kk, ll, i + 1, i + 1, mm, go, ge + 1, ins, ST_D));
                                             }
                                         }
@@ -1538,7 +1552,7 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) k_
                                     for (int j = 1; j <= 4; ++j) {
                                         const int c = (qbase + j) & 3;
                                         const int is_mm = (j != 4 || qbase > 3);
-                                        const uint64_t kk = M.L2[c] + ca[c] + 1, ll = M.L2[c] + cb[c];
+                                        const uint64_t kk = L2s[c] + ca[c] + 1, ll = L2s[c] + cb[c];
                                         if (kk > ll) continue;
                                         if (is_mm) {
                                             logp((mm + 1) * s_mm + go * s_gapo + ge * s_gape,
@@ -1549,7 +1563,7 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) k_
                                         }
                                     }
                                 } else if (qbase < 4) {
-                                    const uint64_t kk = M.L2[qbase] + ca[qbase] + 1, ll = M.L2[qbase] + cb[qbase];
+                                    const uint64_t kk = L2s[qbase] + ca[qbase] + 1, ll = L2s[qbase] + cb[qbase];
                                     if (kk <= ll && kind == K_END) {
                                         child = mk_ent(kk, ll, i, 0, mm, go, ge, ins, ST_M);
                                         cont = true;
@@ -2980,7 +2994,7 @@ int galn_hip_readset_autotune(galn_hip_index *idx, galn_hip_readset *rs, uint32_
         galn_hip_tune_t t{};
         t.sort_by_cost = 1;
         t.block_threads = 128;
-        const int cut[] = {256, 512, 1024, 2048, 4096};
+        const int cut[] = {128, 256, 512, 1024, 2048, 4096};
         int r;
         if (heavy_ok(rs)) {
             for (int c : cut) {
