@@ -216,13 +216,16 @@ __device__ __forceinline__ uint32_t lf_pair(const uint32_t *__restrict__ bwt, co
     load_block(bwt, bb, B);
     if (!aneg && !same) load_block(bwt, ab, A);
     block_occ4(B, (int)(bp & 127), cb);
-    if (aneg) {
-        ca[0] = ca[1] = ca[2] = ca[3] = 0;
-    } else if (same) {
-        block_occ4(B, (int)(ap & 127), ca);
-    } else {
-        block_occ4(A, (int)(ap & 127), ca);
+    // a's block, picked per lane, so that one count serves lanes of both kinds
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+        A[t].x = same ? B[t].x : A[t].x;
+        A[t].y = same ? B[t].y : A[t].y;
+        A[t].z = same ? B[t].z : A[t].z;
+        A[t].w = same ? B[t].w : A[t].w;
     }
+    block_occ4(A, (int)(ap & 127), ca);
+    if (aneg) ca[0] = ca[1] = ca[2] = ca[3] = 0;
     return (aneg || same) ? 64u : 128u;
 }
 
@@ -1225,8 +1228,11 @@ __device__ uint64_t cal_width_one(const uint32_t *bwt, const DMeta &M, int len, 
         const int c = c0 > 3 ? 4 : 3 - c0;
         if (c < 4) {
             bytes += lf_pair(bwt, M, k - 1, l, ca, cb);
-            k = L2s[c] + ca[c] + 1;
-            l = L2s[c] + cb[c];
+            // (selects: a count array indexed by the symbol would live in scratch memory)
+            const uint64_t oa = c == 0 ? ca[0] : c == 1 ? ca[1] : c == 2 ? ca[2] : ca[3];
+            const uint64_t ob = c == 0 ? cb[0] : c == 1 ? cb[1] : c == 2 ? cb[2] : cb[3];
+            k = L2s[c] + oa + 1;
+            l = L2s[c] + ob;
         }
         if (k > l || c > 3) {
             k = 0;
