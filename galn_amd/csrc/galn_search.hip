@@ -1217,7 +1217,7 @@ __device__ __forceinline__ int wave_max_i32(int v)
     return v;
 }
 
-// bwt_cal_width into LDS by one lane (src/bwtgap_hd.h:216-242); returns the algorithmic bytes
+// bwt_cal_width by one lane (src/bwtgap_hd.h:216-242); returns the algorithmic bytes
 __device__ uint64_t cal_width_one(const uint32_t *bwt, const DMeta &M, int len, const uint8_t *str, uint64_t *w)
 {
     const L2Sel L2s(M);
@@ -1243,6 +1243,44 @@ __device__ uint64_t cal_width_one(const uint32_t *bwt, const DMeta &M, int len, 
     }
     w[len] = wpack(0, ++bid);
     return bytes;
+}
+
+// Widths of the reads given to the wave engine whose hand-off slot is empty (a record found by
+// k_search had already changed them): one thread per read, all reads at once, so that the wave
+// engine does not hold 64 lanes while one of them walks a read's sequential width pass.
+struct WArgs {
+    const uint32_t *bwt;
+    DMeta meta;
+    const uint8_t *query;
+    const uint64_t *qoff;
+    const int32_t *qlen;
+    const int32_t *batch;
+    const galn_search_opt_t *bopts;
+    const int32_t *list;
+    int32_t n_list;
+    const int32_t *n_list_dev;
+    uint64_t *hand;
+    uint32_t hand_stride;
+    int32_t hand_cap;
+    int32_t wlen, swlen;
+};
+
+__global__ void __launch_bounds__(256) k_widths(WArgs a)
+{
+    const int n = min(a.n_list_dev ? *a.n_list_dev : a.n_list, a.hand_cap);
+    for (int pos = blockIdx.x * blockDim.x + threadIdx.x; pos < n; pos += gridDim.x * blockDim.x) {
+        uint64_t *const dst = a.hand + (uint64_t)pos * a.hand_stride;
+        if (dst[0] >> 63) continue;  // handed over by k_search
+        const int rid = a.list[pos];
+        const int len = a.qlen[rid];
+        if (len < 1 || len + 1 > a.wlen) continue;  // the wave engine rejects it
+        const int sl = a.bopts[a.batch[rid]].seed_len;
+        if (sl < len && sl + 1 > a.swlen) continue;
+        const uint8_t *const q = a.query + a.qoff[rid];
+        uint64_t wb = cal_width_one(a.bwt, a.meta, len, q, dst + 1);
+        if (sl < len) wb += cal_width_one(a.bwt, a.meta, sl, q + (len - sl), dst + 2 + len);
+        dst[0] = wb | (1ull << 63);
+    }
 }
 
 template <bool TRACE>
@@ -2501,6 +2539,27 @@ int run_heavy(galn_hip_readset *rs, uint32_t flags, const int32_t *list, int n, 
     a.algo_bytes = rs->d_algo + li;
     rs->launches.push_back({tier_id, li, 0.0, 0});
     if (!rs->capturing) HIP_OK(hipEventRecord(rs->lev[2 * li], idx->stream));
+    if (a.hand && a.hand_cap > 0) {
+        WArgs w{};
+        w.bwt = a.bwt;
+        w.meta = a.meta;
+        w.query = a.query;
+        w.qoff = a.qoff;
+        w.qlen = a.qlen;
+        w.batch = a.batch;
+        w.bopts = a.bopts;
+        w.list = list;
+        w.n_list = a.n_list;
+        w.n_list_dev = a.n_list_dev;
+        w.hand = rs->d_hand;
+        w.hand_stride = a.hand_stride;
+        w.hand_cap = a.hand_cap;
+        w.wlen = wlen;
+        w.swlen = swlen;
+        const int wgrid = dev_count ? idx->n_cu * 8 : std::max(1, std::min(idx->n_cu * 8, (n + 255) / 256));
+        hipLaunchKernelGGL(k_widths, dim3(wgrid), dim3(256), 0, idx->stream, w);
+        HIP_OK(hipGetLastError());
+    }
     if (flags & GALN_HIP_TRACE)
         hipLaunchKernelGGL(k_heavy<true>, dim3(waves), dim3(HW), lds, idx->stream, a);
     else
