@@ -1245,6 +1245,21 @@ __device__ uint64_t cal_width_one(const uint32_t *bwt, const DMeta &M, int len, 
     return bytes;
 }
 
+// The segment piece {bucket, first rank, length, pool base} of a round's pushes that holds rank
+// r, searched from piece p on: its index, its end rank, and base - first rank (so that rank r
+// lands at off + r).
+struct PieceAt {
+    int p;
+    uint32_t end, off;
+};
+
+__device__ __forceinline__ PieceAt piece_at(const uint4 *pieces, int p, uint32_t r)
+{
+    while (r >= pieces[p].y + pieces[p].z) ++p;
+    const uint4 pc = pieces[p];
+    return PieceAt{p, pc.y + pc.z, pc.w - pc.y};
+}
+
 // Widths of the reads given to the wave engine whose hand-off slot is empty (a record found by
 // k_search had already changed them): one thread per read, all reads at once, so that the wave
 // engine does not hold 64 lanes while one of them walks a read's sequential width pass.
@@ -1823,7 +1838,11 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) k_
             const long long t_c2 = a.stats ? wall_clock64() : 0;
             {
                 uint32_t q0 = (uint32_t)r0, q1 = (uint32_t)r1, q2 = (uint32_t)r2;
-                const int fp0 = first_piece[tg0], fp1 = first_piece[tg1], fp2 = first_piece[tg2];
+                // per target bucket, the piece that holds the lane's next rank (the ranks only grow)
+                PieceAt P0{0, 0u, 0u}, P1{0, 0u, 0u}, P2{0, 0u, 0u};
+                if (n0) P0 = piece_at(pieces, first_piece[tg0], q0);
+                if (n1) P1 = piece_at(pieces, first_piece[tg1], q1);
+                if (n2) P2 = piece_at(pieces, first_piece[tg2], q2);
                 // four log entries loaded together, then placed (the first four are loaded)
                 for (int u0 = 0; u0 < ncp; u0 += 4) {
                     if (u0 > 0) {
@@ -1843,9 +1862,10 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) k_
                             x = sc == tg0 ? 0 : sc == tg1 ? 1 : 2;  // (3 was ruled out above)
                         }
                         const uint32_t r = x == 0 ? q0++ : x == 1 ? q1++ : q2++;
-                        int p = x == 0 ? fp0 : x == 1 ? fp1 : fp2;
-                        while (r >= pieces[p].y + pieces[p].z) ++p;
-                        POOL[pieces[p].w + (r - pieces[p].y)] = pu[t];
+                        if (x == 0 && r >= P0.end) P0 = piece_at(pieces, P0.p + 1, r);
+                        if (x == 1 && r >= P1.end) P1 = piece_at(pieces, P1.p + 1, r);
+                        if (x == 2 && r >= P2.end) P2 = piece_at(pieces, P2.p + 1, r);
+                        POOL[(x == 0 ? P0.off : x == 1 ? P1.off : P2.off) + r] = pu[t];
                     }
                 }
             }
