@@ -1121,6 +1121,7 @@ constexpr int H_PUSH = 128;    // logged pushes per lane per round
 constexpr int H_CAP = 4096;    // pushes committed per round
 constexpr int H_KEEP = 64;     // granules a wave keeps between reads
 constexpr int H_PIECES = 48;    // segment pieces one round may add (three buckets)
+constexpr int H_GRP = 8;       // logged pushes a lane loads together when placing them
 constexpr uint32_t GENT = (uint32_t)GRAN * CH;  // entries per granule
 
 enum : int { K_END = 0, K_HIT = 1, K_TRUNC = 2, K_ESCORE = 3 };
@@ -1694,9 +1695,9 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) k_
                 ncp += SINFO[t] & 0xff;
             }
             // the first committed pushes are loaded now, to arrive while the plan is applied
-            Ent pu[4];
+            Ent pu[H_GRP];
 #pragma unroll
-            for (int t = 0; t < 4; ++t)
+            for (int t = 0; t < H_GRP; ++t)
                 if (t < ncp) pu[t] = PLOG[t];
             const unsigned long long sum_pops = wave_sum_u64(cj), sum_by = wave_sum_u64(by);
             const int sum_ne = wave_sum_i32(cj > 0 ? ncp - 1 : 0);
@@ -1843,15 +1844,15 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) k_
                 if (n0) P0 = piece_at(pieces, first_piece[tg0], q0);
                 if (n1) P1 = piece_at(pieces, first_piece[tg1], q1);
                 if (n2) P2 = piece_at(pieces, first_piece[tg2], q2);
-                // four log entries loaded together, then placed (the first four are loaded)
-                for (int u0 = 0; u0 < ncp; u0 += 4) {
+                // H_GRP log entries loaded together, then placed (the first group is loaded)
+                for (int u0 = 0; u0 < ncp; u0 += H_GRP) {
                     if (u0 > 0) {
 #pragma unroll
-                        for (int t = 0; t < 4; ++t)
+                        for (int t = 0; t < H_GRP; ++t)
                             if (u0 + t < ncp) pu[t] = PLOG[u0 + t];
                     }
 #pragma unroll
-                    for (int t = 0; t < 4; ++t) {
+                    for (int t = 0; t < H_GRP; ++t) {
                         const int u = u0 + t;
                         if (u >= ncp) break;
                         int x;
