@@ -166,9 +166,11 @@ __device__ __forceinline__ T cold_ld(const __attribute__((address_space(4))) T *
 #define CA(f) cold_ld(&A->f)
 
 // ---------------------------------------------------------------------------
-// Occurrence counts. Index layout: 128-symbol blocks of 16 u32 words, 4 x u64
-// exclusive counts then 8 words of 16 two-bit symbols (first symbol in bits
-// 31-30); position p of the $-inclusive BWT is at p - (p >= primary).
+// Occurrence counts. Device index layout: 128-symbol blocks of 16 u32 words, 4 x u64
+// exclusive counts, then the block's symbols as two bit planes: 4 words of high bits, then 4
+// words of low bits, symbol s at bit 31 - (s & 31) of word s >> 5 of each plane. The .bwt body
+// (BWA's layout: 8 words of 16 two-bit symbols, first symbol in bits 31-30) is uploaded as is and
+// converted in place by k_planes. Position p of the $-inclusive BWT is at p - (p >= primary).
 
 __device__ __forceinline__ void load_block(const uint32_t *__restrict__ bwt, uint64_t blk, uint4 (&b)[4])
 {
@@ -179,26 +181,53 @@ __device__ __forceinline__ void load_block(const uint32_t *__restrict__ bwt, uin
     b[3] = p[3];
 }
 
-// counts of all 4 bases among symbols 0..r of a loaded block, plus its prefix counts
+// counts of all 4 bases among symbols 0..r of a loaded block, plus its prefix counts: per plane
+// word, the high bits count bases 2 and 3, the low bits 1 and 3, both together base 3
 __device__ __forceinline__ void block_occ4(const uint4 (&b)[4], int r, uint64_t c[4])
 {
-    const uint32_t w[8] = {b[2].x, b[2].y, b[2].z, b[2].w, b[3].x, b[3].y, b[3].z, b[3].w};
-    const int last = r >> 4;
-    const uint32_t part = 0x55555555u & ~((1u << (30 - 2 * (r & 15))) - 1u);
-    uint32_t n1 = 0, n2 = 0, n3 = 0;
+    const uint32_t hp[4] = {b[2].x, b[2].y, b[2].z, b[2].w};
+    const uint32_t lp[4] = {b[3].x, b[3].y, b[3].z, b[3].w};
+    uint32_t nh = 0, nl = 0, n3 = 0;
 #pragma unroll
-    for (int j = 0; j < 8; ++j) {
-        const uint32_t valid = j < last ? 0x55555555u : (j == last ? part : 0u);
-        const uint32_t hi = (w[j] >> 1) & valid, lo = w[j] & valid;
-        n3 += __builtin_popcount(hi & lo);
-        n2 += __builtin_popcount(hi & ~lo);
-        n1 += __builtin_popcount(lo & ~hi);
+    for (int j = 0; j < 4; ++j) {
+        // the top t bits of word j: t = symbols of the word at or before r, 0..32
+        const uint32_t t = (uint32_t)min(max(r + 1 - 32 * j, 0), 32);
+        const uint32_t valid = (uint32_t)(0xffffffff00000000ull >> t);
+        const uint32_t h = hp[j] & valid, l = lp[j] & valid;
+        nh += __builtin_popcount(h);
+        nl += __builtin_popcount(l);
+        n3 += __builtin_popcount(h & l);
     }
+    const uint32_t n2 = nh - n3, n1 = nl - n3;
     const uint32_t n0 = (uint32_t)(r + 1) - n1 - n2 - n3;
     c[0] = ((uint64_t)b[0].y << 32 | b[0].x) + n0;
     c[1] = ((uint64_t)b[0].w << 32 | b[0].z) + n1;
     c[2] = ((uint64_t)b[1].y << 32 | b[1].x) + n2;
     c[3] = ((uint64_t)b[1].w << 32 | b[1].z) + n3;
+}
+
+// In-place conversion of the uploaded .bwt body's symbol words to bit planes, one block per
+// thread (a last, shorter block included: its missing words were zero-filled).
+__global__ void __launch_bounds__(256) k_planes(uint32_t *bwt, uint64_t n_blocks)
+{
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n_blocks) return;
+    uint4 *const p = reinterpret_cast<uint4 *>(bwt + (i << 4));
+    const uint4 s0 = p[2], s1 = p[3];
+    const uint32_t w[8] = {s0.x, s0.y, s0.z, s0.w, s1.x, s1.y, s1.z, s1.w};
+    uint32_t hp[4] = {0, 0, 0, 0}, lp[4] = {0, 0, 0, 0};
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+#pragma unroll
+        for (int t = 0; t < 16; ++t) {
+            const int sym = k * 16 + t;  // symbol index in the block
+            const uint32_t c = (w[k] >> (30 - 2 * t)) & 3u;
+            hp[sym >> 5] |= (c >> 1) << (31 - (sym & 31));
+            lp[sym >> 5] |= (c & 1u) << (31 - (sym & 31));
+        }
+    }
+    p[2] = make_uint4(hp[0], hp[1], hp[2], hp[3]);
+    p[3] = make_uint4(lp[0], lp[1], lp[2], lp[3]);
 }
 
 // Occ of all four bases at positions a (may be -1) and b >= 0: bwt_2occ4, src/bwtgap_hd.h:158-192.
@@ -2960,8 +2989,16 @@ int galn_hip_index_open(const uint32_t *bwt, const galn_bwt_meta_t *meta, int de
     }
     if (hipStreamCreateWithFlags(&idx->stream, hipStreamNonBlocking) != hipSuccess ||
         hipMemsetAsync(idx->d_bwt, 0, alloc, idx->stream) != hipSuccess ||
-        hipMemcpyAsync(idx->d_bwt, bwt, bytes, hipMemcpyHostToDevice, idx->stream) != hipSuccess ||
-        hipStreamSynchronize(idx->stream) != hipSuccess) {
+        hipMemcpyAsync(idx->d_bwt, bwt, bytes, hipMemcpyHostToDevice, idx->stream) != hipSuccess) {
+        hipFree(idx->d_bwt);
+        delete idx;
+        return fail(GALN_HIP_E_DEVICE, "index upload failed");
+    }
+    const uint64_t n_blocks = ((uint64_t)meta->bwt_size + 15) / 16;
+    if (n_blocks > 0)
+        hipLaunchKernelGGL(k_planes, dim3((unsigned)((n_blocks + 255) / 256)), dim3(256), 0, idx->stream, idx->d_bwt,
+                           n_blocks);
+    if (hipGetLastError() != hipSuccess || hipStreamSynchronize(idx->stream) != hipSuccess) {
         hipFree(idx->d_bwt);
         delete idx;
         return fail(GALN_HIP_E_DEVICE, "index upload failed");
