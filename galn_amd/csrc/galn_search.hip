@@ -1516,6 +1516,7 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) k_
             const int msd = F.msd, seed_len = F.seed_len;
             const bool seeding = F.seeding != 0;
             int steps = 0, np = 0, kind = K_END;
+            uint32_t sby = 0;    // index bytes of the chain's steps
             uint64_t pcode = 0;  // bucket codes of the first 32 logged pushes
             uint64_t hk = 0, hl = 0;
             Ent nxt{0, 0};
@@ -1662,6 +1663,7 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) k_
                         }
                     }
                     SINFO[steps] = sp | sbytes << 8;
+                    sby += sbytes;
                     ++steps;
                     if (kind != K_END || !cont) break;
                     e = child;
@@ -1671,6 +1673,7 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) k_
             if (lane >= K) {
                 steps = 0;
                 np = 0;
+                sby = 0;
                 kind = K_END;
             }
             const long long t_commit = a.stats ? wall_clock64() : 0;
@@ -1695,14 +1698,18 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) k_
                     c = 0;
                     while (c < steps && acc + (int)(SINFO[c] & 0xff) <= H_CAP) acc += SINFO[c++] & 0xff;
                 }
-                // the max_entries check before each pop (src/bwtgap_hd.h:360)
-                long nb = nstart;
-                for (int t = 0; t < c; ++t) {
-                    if (nb > F.max_entries) {
-                        cut = t;
-                        break;
+                // the max_entries check before each pop (src/bwtgap_hd.h:360): the entries before a
+                // pop are nstart plus the chain's earlier pushes, so only a chain whose pushes could
+                // cross the limit is walked
+                if ((long)nstart + np > F.max_entries) {
+                    long nb = nstart;
+                    for (int t = 0; t < c; ++t) {
+                        if (nb > F.max_entries) {
+                            cut = t;
+                            break;
+                        }
+                        nb += SINFO[t] & 0xff;
                     }
-                    nb += SINFO[t] & 0xff;
                 }
                 if (c > 0 && c == steps && (kind == K_HIT || kind == K_ESCORE)) {
                     F.hk = hk;
@@ -1719,9 +1726,14 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) k_
             const int cj = lane <= last ? (lane == jcut ? cut : c) : 0;  // committed steps of this chain
             int ncp = 0;  // committed pushes of this chain
             uint64_t by = 0;
-            for (int t = 0; t < cj; ++t) {
-                by += SINFO[t] >> 8;
-                ncp += SINFO[t] & 0xff;
+            if (cj == steps) {  // the whole chain
+                ncp = np;
+                by = sby;
+            } else {
+                for (int t = 0; t < cj; ++t) {
+                    by += SINFO[t] >> 8;
+                    ncp += SINFO[t] & 0xff;
+                }
             }
             // the first committed pushes are loaded now, to arrive while the plan is applied
             Ent pu[H_GRP];
