@@ -1518,6 +1518,7 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) k_
             int steps = 0, np = 0, kind = K_END;
             uint32_t sby = 0;    // index bytes of the chain's steps
             uint64_t pcode = 0;  // bucket codes of the first 32 logged pushes
+            uint32_t hcnt = 0;   // pushes after the 32nd per bucket code, 8 bits each
             uint64_t hk = 0, hl = 0;
             Ent nxt{0, 0};
             if (lane < K) {
@@ -1607,10 +1608,12 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) k_
                                     PLOG[np] = en;
                                     // its bucket as the commit numbers them: 0 b+s_mm, 1 b+s_gapo,
                                     // 2 b+s_gape, 3 none of them (not possible with positive penalties)
-                                    if (np < 32) {
-                                        const int d = s - b;
-                                        pcode |= (uint64_t)(d == s_mm ? 0 : d == s_gapo ? 1 : d == s_gape ? 2 : 3) << (2 * np);
-                                    }
+                                    const int d = s - b;
+                                    const int code = d == s_mm ? 0 : d == s_gapo ? 1 : d == s_gape ? 2 : 3;
+                                    if (np < 32)
+                                        pcode |= (uint64_t)code << (2 * np);
+                                    else
+                                        hcnt += 1u << (8 * code);
                                     ++np;
                                     ++sp;
                                 };
@@ -1795,7 +1798,13 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) k_
             const uint64_t clo = pcode & vmask, chi = (pcode >> 1) & vmask;
             int n0 = __builtin_popcountll(vmask & ~clo & ~chi), n1 = __builtin_popcountll(clo & ~chi);
             int n2 = __builtin_popcountll(chi & ~clo), bad = (clo & chi) != 0;
-            for (int u0 = 32; u0 < ncp; u0 += 4) {
+            if (ncp > 32 && ncp == np) {  // the whole log: counted while it was written
+                n0 += hcnt & 0xff;
+                n1 += (hcnt >> 8) & 0xff;
+                n2 += (hcnt >> 16) & 0xff;
+                bad |= (hcnt >> 24) != 0;
+            }
+            for (int u0 = 32; u0 < ncp && ncp < np; u0 += 4) {
                 Ent pv[4];
 #pragma unroll
                 for (int t = 0; t < 4; ++t)
