@@ -148,7 +148,8 @@ void galn_hip_get_limits(galn_hip_limits_t *lim);
 /* Last error text of this thread (static storage). */
 const char *galn_hip_last_error(void);
 
-/* Upload a .bwt body (bwt_size words, as load_bwt returns it) to `device`. */
+/* Upload a .bwt body (bwt_size words, as load_bwt returns it) to `device`; the device copy keeps
+   each block's symbols as two bit planes (the host buffer is not changed). */
 int galn_hip_index_open(const uint32_t *bwt, const galn_bwt_meta_t *meta, int device,
                         galn_hip_index **out);
 /* Returns GALN_HIP_E_BUSY (and keeps the index) while read sets or servers of it are open. */
