@@ -1204,7 +1204,7 @@ struct HShared {
     int has_pend, claim, status, result, K, b, J, jcap, jcut, tcut, n_piece, n_own, n_used, aleft;
     uint32_t abase;
     int nN;
-    int l_partial, l_full;  // the last committed chain: cut inside, or its stop kind when whole (-1)
+    int l_full;             // the last committed chain's stop kind
     unsigned long long st[16];
     uint64_t hk, hl;
     Ent pend, he;
@@ -1529,7 +1529,7 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) k_
                         nxt = e;
                         break;
                     }
-                    SLOG[steps] = e;
+                    if (TRACE) SLOG[steps] = e;  // the trace hash runs over every committed pop
                     uint32_t sp = 0, sbytes = 0;
                     bool cont = false;
                     Ent child{0, 0};
@@ -1665,6 +1665,7 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) k_
                             }
                         }
                     }
+                    if (!TRACE && (kind == K_HIT || kind == K_ESCORE)) SLOG[steps] = e;  // read by the commit
                     SINFO[steps] = sp | sbytes << 8;
                     sby += sbytes;
                     ++steps;
@@ -1689,18 +1690,14 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) k_
             // entries before this chain's first pop: each chain pops its head, pushes its logged
             // children and, when truncated, leaves its next exact child pending
             const int nstart = F.n_entries + wave_excl_scan(inJ ? np - 1 + (kind == K_TRUNC ? 1 : 0) : 0);
-            // the push cap falls in chain jcap, which commits whole steps up to it
+            // the push cap falls in chain jcap, which waits for the next round with the chains
+            // after it (chain 0 always fits: a chain logs at most H_PUSH pushes)
             const uint64_t capm = __ballot(inJ && pex + np > H_CAP);
             const int jcap = capm ? __builtin_ctzll(capm) : J + 1;
-            const bool cand = inJ && lane <= jcap;
+            const bool cand = inJ && lane < jcap;
             int c = 0, cut = -1;
             if (cand) {
                 c = steps;
-                if (lane == jcap) {
-                    int acc = pex;
-                    c = 0;
-                    while (c < steps && acc + (int)(SINFO[c] & 0xff) <= H_CAP) acc += SINFO[c++] & 0xff;
-                }
                 // the max_entries check before each pop (src/bwtgap_hd.h:360): the entries before a
                 // pop are nstart plus the chain's earlier pushes, so only a chain whose pushes could
                 // cross the limit is walked
@@ -1749,11 +1746,8 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) k_
                 lcommit[lane] = cj;
                 lnstart[lane] = nstart;
             }
-            if (lane == last) {
-                F.l_partial = c < steps;
-                F.l_full = c == steps ? kind : -1;
-                if (c < steps) F.pend = SLOG[c];
-            }
+            // a committed chain commits whole (a cut at max_entries ends the search instead)
+            if (lane == last) F.l_full = kind;
             __syncthreads();
             const long long t_c1 = a.stats ? wall_clock64() : 0;
             if (lane == 0) {
@@ -1944,7 +1938,7 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) k_
                 // entries: every committed pop -1, every committed push +1, a pending child +1
                 long ne = F.n_entries + sum_ne;
                 const int lfull = last >= 0 ? F.l_full : -1;
-                if (last >= 0 && (F.l_partial || lfull == K_TRUNC)) {
+                if (last >= 0 && lfull == K_TRUNC) {
                     F.has_pend = 1;
                     ne += 1;
                 }
