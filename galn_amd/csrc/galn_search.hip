@@ -1275,21 +1275,6 @@ __device__ uint64_t cal_width_one(const uint32_t *bwt, const DMeta &M, int len, 
     return bytes;
 }
 
-// The segment piece {bucket, first rank, length, pool base} of a round's pushes that holds rank
-// r, searched from piece p on: its index, its end rank, and base - first rank (so that rank r
-// lands at off + r).
-struct PieceAt {
-    int p;
-    uint32_t end, off;
-};
-
-__device__ __forceinline__ PieceAt piece_at(const uint4 *pieces, int p, uint32_t r)
-{
-    while (r >= pieces[p].y + pieces[p].z) ++p;
-    const uint4 pc = pieces[p];
-    return PieceAt{p, pc.y + pc.z, pc.w - pc.y};
-}
-
 // Widths of the reads given to the wave engine whose hand-off slot is empty (a record found by
 // k_search had already changed them): one thread per read, all reads at once, so that the wave
 // engine does not hold 64 lanes while one of them walks a read's sequential width pass.
@@ -1735,11 +1720,6 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) k_
                     ncp += SINFO[t] & 0xff;
                 }
             }
-            // the first committed pushes are loaded now, to arrive while the plan is applied
-            Ent pu[H_GRP];
-#pragma unroll
-            for (int t = 0; t < H_GRP; ++t)
-                if (t < ncp) pu[t] = PLOG[t];
             const unsigned long long sum_pops = wave_sum_u64(cj), sum_by = wave_sum_u64(by);
             const int sum_ne = wave_sum_i32(cj > 0 ? ncp - 1 : 0);
             if (TRACE) {
@@ -1870,6 +1850,7 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) k_
                     seg_cnt[sb] = prev_size;
                     F.bm[sb >> 6] |= 1ull << (sb & 63);
                 }
+                F.n_piece = np_;
                 if (badw) {  // a push outside the three buckets: not possible with positive penalties
                     F.status = 1;
                     F.result = ERR_INTERNAL;
@@ -1882,40 +1863,77 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) k_
             if (F.status) break;
             const long long t_c2 = a.stats ? wall_clock64() : 0;
             {
-                uint32_t q0 = (uint32_t)r0, q1 = (uint32_t)r1, q2 = (uint32_t)r2;
-                // per target bucket, the piece that holds the lane's next rank (the ranks only grow)
-                PieceAt P0{0, 0u, 0u}, P1{0, 0u, 0u}, P2{0, 0u, 0u};
-                if (n0) P0 = piece_at(pieces, first_piece[tg0], q0);
-                if (n1) P1 = piece_at(pieces, first_piece[tg1], q1);
-                if (n2) P2 = piece_at(pieces, first_piece[tg2], q2);
-                // H_GRP log entries loaded together, then placed (the first group is loaded)
-                for (int u0 = 0; u0 < ncp; u0 += H_GRP) {
-                    if (u0 > 0) {
+                const int npieces = F.n_piece;
+                const uint64_t ODD = 0x5555555555555555ull;
+                const Ent *const PLOG_W = reinterpret_cast<const Ent *>(base + a.plog_off);  // lane j's log at j * H_PUSH
+                bool bad_piece = false;
+                // rank R of bucket tg's pushes this round to its pool slot, within this round's pieces
+                auto place = [&](int tg, uint32_t R, const Ent &pe) __attribute__((always_inline)) {
+                    int p = first_piece[tg];
+                    while (p < npieces && R >= pieces[p].y + pieces[p].z) ++p;
+                    if (p < npieces && (int)pieces[p].x == tg)
+                        POOL[pieces[p].w + (R - pieces[p].y)] = pe;
+                    else
+                        bad_piece = true;
+                };
+                // a lane's first 32 pushes, spread over the wave: push g of the round (lane order,
+                // then log order) is placed by lane g % 64, its bucket and rank read from its owner
+                // lane's 2-bit codes and bucket offsets
+                const int m32 = min(ncp, 32);
+                const int E = wave_excl_scan(m32);
+                const int M = __shfl(E + m32, HW - 1);
+                const uint32_t pc_lo = (uint32_t)pcode, pc_hi = (uint32_t)(pcode >> 32);
+                for (int g0 = 0; g0 < M; g0 += HW) {
+                    const int g = g0 + lane;
+                    int L = 0;  // the owner: the last lane whose first push is at or before g
+#pragma unroll
+                    for (int step = 32; step > 0; step >>= 1) {
+                        const int el = __shfl(E, L + step);
+                        if (el <= g) L += step;
+                    }
+                    const int u = g - __shfl(E, L);
+                    const uint64_t pc = (uint64_t)__shfl(pc_hi, L) << 32 | __shfl(pc_lo, L);
+                    const int ra = __shfl(r0, L), rb = __shfl(r1, L), rc = __shfl(r2, L);
+                    if (g < M && (u < 0 || u >= 32)) bad_piece = true;  // (not possible: E is a scan)
+                    if (g < M && u >= 0 && u < 32) {
+                        const int x = (int)(pc >> (2 * u)) & 3;
+                        // the owner's earlier pushes with the same code: fields of pc equal to x
+                        const uint64_t d = pc ^ (x == 0 ? 0ull : x == 1 ? ODD : x == 2 ? (ODD << 1) : ~0ull);
+                        const uint64_t same = ~(d | (d >> 1)) & ODD;
+                        const uint64_t below = u ? (~0ull >> (64 - 2 * u)) : 0ull;
+                        const uint32_t R = (uint32_t)((x == 0 ? ra : x == 1 ? rb : rc) + __builtin_popcountll(same & below));
+                        place(x == 0 ? tg0 : x == 1 ? tg1 : tg2, R, PLOG_W[L * H_PUSH + u]);
+                    }
+                }
+                // pushes after a lane's 32nd: by the lane itself, ranks continuing after its first 32
+                if (ncp > 32) {
+                    const uint64_t lo_b = pcode & ODD, hi_b = (pcode >> 1) & ODD;
+                    uint32_t q0 = (uint32_t)(r0 + __builtin_popcountll(ODD & ~lo_b & ~hi_b));
+                    uint32_t q1 = (uint32_t)(r1 + __builtin_popcountll(lo_b & ~hi_b));
+                    uint32_t q2 = (uint32_t)(r2 + __builtin_popcountll(hi_b & ~lo_b));
+                    for (int u0 = 32; u0 < ncp; u0 += H_GRP) {
+                        Ent pu[H_GRP];
 #pragma unroll
                         for (int t = 0; t < H_GRP; ++t)
                             if (u0 + t < ncp) pu[t] = PLOG[u0 + t];
-                    }
 #pragma unroll
-                    for (int t = 0; t < H_GRP; ++t) {
-                        const int u = u0 + t;
-                        if (u >= ncp) break;
-                        int x;
-                        if (u < 32) {
-                            x = (int)(pcode >> (2 * u)) & 3;
-                        } else {
+                        for (int t = 0; t < H_GRP; ++t) {
+                            if (u0 + t >= ncp) break;
                             const int sc = e_mm(pu[t]) * s_mm + e_go(pu[t]) * s_gapo + e_ge(pu[t]) * s_gape;
-                            x = sc == tg0 ? 0 : sc == tg1 ? 1 : 2;  // (3 was ruled out above)
+                            const int x = sc == tg0 ? 0 : sc == tg1 ? 1 : 2;  // (3 was ruled out above)
+                            const uint32_t R = x == 0 ? q0++ : x == 1 ? q1++ : q2++;
+                            place(x == 0 ? tg0 : x == 1 ? tg1 : tg2, R, pu[t]);
                         }
-                        const uint32_t r = x == 0 ? q0++ : x == 1 ? q1++ : q2++;
-                        if (x == 0 && r >= P0.end) P0 = piece_at(pieces, P0.p + 1, r);
-                        if (x == 1 && r >= P1.end) P1 = piece_at(pieces, P1.p + 1, r);
-                        if (x == 2 && r >= P2.end) P2 = piece_at(pieces, P2.p + 1, r);
-                        POOL[(x == 0 ? P0.off : x == 1 ? P1.off : P2.off) + r] = pu[t];
                     }
+                }
+                if (bad_piece) {  // a rank outside this round's pieces: not possible
+                    F.status = 1;
+                    F.result = ERR_INTERNAL;
                 }
             }
             const int mxs = a.stats ? wave_max_i32(steps) : 0;
             __syncthreads();
+            if (F.status) break;
             const long long t_c3 = a.stats ? wall_clock64() : 0;
             // ---- bucket b: the committed start entries leave its top; the state moves on ----
             if (lane == 0) {
