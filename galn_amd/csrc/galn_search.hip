@@ -1314,7 +1314,9 @@ __global__ void __launch_bounds__(256) k_widths(WArgs a)
 }
 
 template <bool TRACE>
-__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) k_heavy(HArgs a)
+// 4 waves per SIMD (16 per CU): at 128 VGPRs the compiler spills a few values to scratch, and
+// the extra resident waves hide more of the index latency than the spills cost (measured 3.4%)
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) k_heavy(HArgs a)
 {
     extern __shared__ __align__(16) uint8_t hlds[];
     HShared &F = *reinterpret_cast<HShared *>(hlds);
@@ -1330,7 +1332,9 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) k_
     uint64_t *SW = W + a.wlen;
     int32_t *lcommit = reinterpret_cast<int32_t *>(SW + a.swlen);  // per chain, for the trace hash
     int32_t *lnstart = lcommit + HW;
-    uint32_t *const SINFO_ALL = reinterpret_cast<uint32_t *>(lnstart + HW);  // per chain step: pushes | bytes << 8
+    // per chain step: pushes (4 bits) | index bytes / 64 << 4 (a step's bytes are whole 64-byte
+    // blocks, at most 2 per position of a read of at most 1023: under 4096 units)
+    uint16_t *const SINFO_ALL = reinterpret_cast<uint16_t *>(lnstart + HW);
     uint8_t *const QL = reinterpret_cast<uint8_t *>(SINFO_ALL + HW * H_STEPS);  // the read's query
 
     const int lane = threadIdx.x;
@@ -1341,7 +1345,8 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) k_
     Ent *const PLOG = reinterpret_cast<Ent *>(base + a.plog_off) + lane * H_PUSH;
     Ent *const SLOG_ALL = reinterpret_cast<Ent *>(base + a.slog_off);
     Ent *const SLOG = SLOG_ALL + lane * H_STEPS;
-    uint32_t *const SINFO = SINFO_ALL + lane * H_STEPS;
+    uint16_t *const SINFO = SINFO_ALL + lane * H_STEPS;
+    static_assert(MAXLEN * 128 / 64 < 4096, "a step's index bytes fit the step info");
     Ent *const POOL = a.pool;
     const DMeta &M = a.meta;
     const L2Sel L2s(M);
@@ -1651,7 +1656,7 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) k_
                         }
                     }
                     if (!TRACE && (kind == K_HIT || kind == K_ESCORE)) SLOG[steps] = e;  // read by the commit
-                    SINFO[steps] = sp | sbytes << 8;
+                    SINFO[steps] = (uint16_t)(sp | (sbytes >> 6) << 4);
                     sby += sbytes;
                     ++steps;
                     if (kind != K_END || !cont) break;
@@ -1693,7 +1698,7 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) k_
                             cut = t;
                             break;
                         }
-                        nb += SINFO[t] & 0xff;
+                        nb += SINFO[t] & 0xf;
                     }
                 }
                 if (c > 0 && c == steps && (kind == K_HIT || kind == K_ESCORE)) {
@@ -1716,8 +1721,8 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) k_
                 by = sby;
             } else {
                 for (int t = 0; t < cj; ++t) {
-                    by += SINFO[t] >> 8;
-                    ncp += SINFO[t] & 0xff;
+                    by += (uint64_t)(SINFO[t] >> 4) << 6;
+                    ncp += SINFO[t] & 0xf;
                 }
             }
             const unsigned long long sum_pops = wave_sum_u64(cj), sum_by = wave_sum_u64(by);
@@ -1748,7 +1753,7 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) k_
                             tr = tr_mix(tr, (uint64_t)ins | (uint64_t)(go + ge - ins) << 16);
                             tr = tr_mix(tr, (uint64_t)(uint32_t)e_ldp(e));
                             tr = tr_mix(tr, (uint64_t)(uint32_t)nb);
-                            nb += (info & 0xff) + 1;  // its pushes, and its exact child (popped next, if any)
+                            nb += (info & 0xf) + 1;  // its pushes, and its exact child (popped next, if any)
                         }
                     }
                     F.trace = tr;
@@ -2374,7 +2379,7 @@ size_t heavy_lds_bytes(int buckets, int wlen, int swlen)
     size_t b = (sizeof(HShared) + 15) & ~(size_t)15;
     b += (size_t)buckets * 12;
     b = (b + 15) & ~(size_t)15;
-    b += (size_t)H_PIECES * 16 + (size_t)(wlen + swlen) * 8 + 2 * HW * 4 + (size_t)HW * H_STEPS * 4 + (size_t)wlen;
+    b += (size_t)H_PIECES * 16 + (size_t)(wlen + swlen) * 8 + 2 * HW * 4 + (size_t)HW * H_STEPS * 2 + (size_t)wlen;
     return b;
 }
 
