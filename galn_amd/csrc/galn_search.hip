@@ -1869,7 +1869,6 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) k_
             const long long t_c2 = a.stats ? wall_clock64() : 0;
             {
                 const int npieces = F.n_piece;
-                const uint64_t ODD = 0x5555555555555555ull;
                 const Ent *const PLOG_W = reinterpret_cast<const Ent *>(base + a.plog_off);  // lane j's log at j * H_PUSH
                 bool bad_piece = false;
                 // rank R of bucket tg's pushes this round to its pool slot, within this round's pieces
@@ -1881,13 +1880,14 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) k_
                     else
                         bad_piece = true;
                 };
-                // a lane's first 32 pushes, spread over the wave: push g of the round (lane order,
-                // then log order) is placed by lane g % 64, its bucket and rank read from its owner
-                // lane's 2-bit codes and bucket offsets
-                const int m32 = min(ncp, 32);
-                const int E = wave_excl_scan(m32);
-                const int M = __shfl(E + m32, HW - 1);
-                const uint32_t pc_lo = (uint32_t)pcode, pc_hi = (uint32_t)(pcode >> 32);
+                // every committed push, spread over the wave: push g of the round in the sequential
+                // order (chain by chain, each in log order) is placed by lane g % 64. Its rank in
+                // its bucket is the number of earlier pushes to that bucket in this order: a ballot
+                // per bucket within a spread, plus the counts of the spreads before it.
+                const int E = wave_excl_scan(ncp);
+                const int M = __shfl(E + ncp, HW - 1);
+                const uint64_t lt = (1ull << lane) - 1ull;  // the lanes before this one
+                uint32_t c0 = 0, c1 = 0, c2 = 0;
                 for (int g0 = 0; g0 < M; g0 += HW) {
                     const int g = g0 + lane;
                     int L = 0;  // the owner: the last lane whose first push is at or before g
@@ -1897,39 +1897,29 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) k_
                         if (el <= g) L += step;
                     }
                     const int u = g - __shfl(E, L);
-                    const uint64_t pc = (uint64_t)__shfl(pc_hi, L) << 32 | __shfl(pc_lo, L);
-                    const int ra = __shfl(r0, L), rb = __shfl(r1, L), rc = __shfl(r2, L);
-                    if (g < M && (u < 0 || u >= 32)) bad_piece = true;  // (not possible: E is a scan)
-                    if (g < M && u >= 0 && u < 32) {
-                        const int x = (int)(pc >> (2 * u)) & 3;
-                        // the owner's earlier pushes with the same code: fields of pc equal to x
-                        const uint64_t d = pc ^ (x == 0 ? 0ull : x == 1 ? ODD : x == 2 ? (ODD << 1) : ~0ull);
-                        const uint64_t same = ~(d | (d >> 1)) & ODD;
-                        const uint64_t below = u ? (~0ull >> (64 - 2 * u)) : 0ull;
-                        const uint32_t R = (uint32_t)((x == 0 ? ra : x == 1 ? rb : rc) + __builtin_popcountll(same & below));
-                        place(x == 0 ? tg0 : x == 1 ? tg1 : tg2, R, PLOG_W[L * H_PUSH + u]);
+                    const bool valid = g < M;
+                    if (valid && (u < 0 || u >= H_PUSH)) bad_piece = true;  // (not possible: E is a scan)
+                    Ent pe{0, 0};
+                    int x = 3;
+                    if (valid && u >= 0 && u < H_PUSH) {
+                        pe = PLOG_W[L * H_PUSH + u];
+                        const int sc = e_mm(pe) * s_mm + e_go(pe) * s_gapo + e_ge(pe) * s_gape;  // its bucket
+                        x = sc == tg0 ? 0 : sc == tg1 ? 1 : sc == tg2 ? 2 : 3;
                     }
-                }
-                // pushes after a lane's 32nd: by the lane itself, ranks continuing after its first 32
-                if (ncp > 32) {
-                    const uint64_t lo_b = pcode & ODD, hi_b = (pcode >> 1) & ODD;
-                    uint32_t q0 = (uint32_t)(r0 + __builtin_popcountll(ODD & ~lo_b & ~hi_b));
-                    uint32_t q1 = (uint32_t)(r1 + __builtin_popcountll(lo_b & ~hi_b));
-                    uint32_t q2 = (uint32_t)(r2 + __builtin_popcountll(hi_b & ~lo_b));
-                    for (int u0 = 32; u0 < ncp; u0 += H_GRP) {
-                        Ent pu[H_GRP];
-#pragma unroll
-                        for (int t = 0; t < H_GRP; ++t)
-                            if (u0 + t < ncp) pu[t] = PLOG[u0 + t];
-#pragma unroll
-                        for (int t = 0; t < H_GRP; ++t) {
-                            if (u0 + t >= ncp) break;
-                            const int sc = e_mm(pu[t]) * s_mm + e_go(pu[t]) * s_gapo + e_ge(pu[t]) * s_gape;
-                            const int x = sc == tg0 ? 0 : sc == tg1 ? 1 : 2;  // (3 was ruled out above)
-                            const uint32_t R = x == 0 ? q0++ : x == 1 ? q1++ : q2++;
-                            place(x == 0 ? tg0 : x == 1 ? tg1 : tg2, R, pu[t]);
+                    const uint64_t b0 = __ballot(x == 0), b1 = __ballot(x == 1), b2 = __ballot(x == 2);
+                    if (valid && u >= 0 && u < H_PUSH) {
+                        if (x == 3) {
+                            bad_piece = true;  // (ruled out when the round was sized)
+                        } else {
+                            const uint32_t R = x == 0   ? c0 + (uint32_t)__builtin_popcountll(b0 & lt)
+                                               : x == 1 ? c1 + (uint32_t)__builtin_popcountll(b1 & lt)
+                                                        : c2 + (uint32_t)__builtin_popcountll(b2 & lt);
+                            place(x == 0 ? tg0 : x == 1 ? tg1 : tg2, R, pe);
                         }
                     }
+                    c0 += (uint32_t)__builtin_popcountll(b0);
+                    c1 += (uint32_t)__builtin_popcountll(b1);
+                    c2 += (uint32_t)__builtin_popcountll(b2);
                 }
                 if (bad_piece) {  // a rank outside this round's pieces: not possible
                     F.status = 1;
