@@ -1507,8 +1507,7 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) k_
             const bool seeding = F.seeding != 0;
             int steps = 0, np = 0, kind = K_END;
             uint32_t sby = 0;    // index bytes of the chain's steps
-            uint64_t pcode = 0;  // bucket codes of the first 32 logged pushes
-            uint32_t hcnt = 0;   // pushes after the 32nd per bucket code, 8 bits each
+            uint32_t ccnt = 0;   // logged pushes per bucket code, 8 bits each
             uint64_t hk = 0, hl = 0;
             Ent nxt{0, 0};
             if (lane < K) {
@@ -1600,10 +1599,7 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) k_
                                     // 2 b+s_gape, 3 none of them (not possible with positive penalties)
                                     const int d = s - b;
                                     const int code = d == s_mm ? 0 : d == s_gapo ? 1 : d == s_gape ? 2 : 3;
-                                    if (np < 32)
-                                        pcode |= (uint64_t)code << (2 * np);
-                                    else
-                                        hcnt += 1u << (8 * code);
+                                    ccnt += 1u << (8 * code);  // at most H_PUSH of each
                                     ++np;
                                     ++sp;
                                 };
@@ -1771,36 +1767,12 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) k_
             // (a mismatch, a gap open, a gap extension); its rank in the bucket is its place in
             // the sequential push order, chain by chain ----
             const int tg0 = b + s_mm, tg1 = b + s_gapo, tg2 = b + s_gape;
-            // pushes per bucket: the first 32 from their 2-bit codes, the rest from the log
-            const uint64_t ODD = 0x5555555555555555ull;
-            const uint64_t vmask = ncp >= 32 ? ODD : (((uint64_t)1 << (2 * ncp)) - 1) & ODD;
-            const uint64_t clo = pcode & vmask, chi = (pcode >> 1) & vmask;
-            int n0 = __builtin_popcountll(vmask & ~clo & ~chi), n1 = __builtin_popcountll(clo & ~chi);
-            int n2 = __builtin_popcountll(chi & ~clo), bad = (clo & chi) != 0;
-            if (ncp > 32 && ncp == np) {  // the whole log: counted while it was written
-                n0 += hcnt & 0xff;
-                n1 += (hcnt >> 8) & 0xff;
-                n2 += (hcnt >> 16) & 0xff;
-                bad |= (hcnt >> 24) != 0;
-            }
-            for (int u0 = 32; u0 < ncp && ncp < np; u0 += 4) {
-                Ent pv[4];
-#pragma unroll
-                for (int t = 0; t < 4; ++t)
-                    if (u0 + t < ncp) pv[t] = PLOG[u0 + t];
-#pragma unroll
-                for (int t = 0; t < 4; ++t) {
-                    if (u0 + t >= ncp) break;
-                    const int sc = e_mm(pv[t]) * s_mm + e_go(pv[t]) * s_gapo + e_ge(pv[t]) * s_gape;  // its bucket
-                    const int x = sc == tg0 ? 0 : sc == tg1 ? 1 : sc == tg2 ? 2 : 3;
-                    n0 += x == 0;
-                    n1 += x == 1;
-                    n2 += x == 2;
-                    bad |= x == 3;
-                }
-            }
-            const int r0 = wave_excl_scan(n0), r1 = wave_excl_scan(n1), r2 = wave_excl_scan(n2);
-            const int T0 = __shfl(r0 + n0, HW - 1), T1 = __shfl(r1 + n1, HW - 1), T2 = __shfl(r2 + n2, HW - 1);
+            // pushes per bucket, counted while the log was written: a round commits a chain whole,
+            // or ends the search (max_entries) before its pushes are placed
+            const uint32_t cc = ncp > 0 ? ccnt : 0u;
+            const int n0 = (int)(cc & 0xff), n1 = (int)((cc >> 8) & 0xff), n2 = (int)((cc >> 16) & 0xff);
+            const bool bad = (cc >> 24) != 0 || (ncp != 0 && ncp != np);
+            const int T0 = wave_sum_i32(n0), T1 = wave_sum_i32(n1), T2 = wave_sum_i32(n2);
             const bool badw = __ballot(bad) != 0;
             if (lane == 0) {
                 // pieces of the wave's current granule for each bucket that receives pushes; a
