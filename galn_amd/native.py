@@ -408,17 +408,49 @@ class ReadSet:
         hip_lib().galn_hip_readset_kernel_ms(self.h, C.byref(a), C.byref(b))
         return a.value, b.value
 
-    def results(self, want_pops=True, want_trace=False, want_bytes=False, rec_cap: int | None = None) -> Results:
+    def result_buffers(self, want_pops=True, want_trace=False, want_bytes=False, rec_cap: int | None = None,
+                       alloc=np.empty) -> dict:
+        """Arrays for `results(out=...)`, to be reused across searches of this read set;
+        `alloc(shape, dtype)` makes each one (e.g. in pinned host memory)."""
+        n1 = max(self.n, 1)
+        out = {"n_aln": alloc((n1,), np.int32), "rec_off": alloc((n1,), np.uint64),
+               "rec": alloc((max(rec_cap if rec_cap is not None else 2 * self.n, 1), 3), np.uint64)}
+        for k, want in (("pops", want_pops), ("trace", want_trace), ("bytes", want_bytes)):
+            if want:
+                out[k] = alloc((n1,), np.uint64)
+        return out
+
+    def results(self, want_pops=True, want_trace=False, want_bytes=False, rec_cap: int | None = None,
+                out: dict | None = None) -> Results:
+        """Records in read order. With `out` (from `result_buffers`) the arrays are filled in
+        place, and the returned views share them until the next call."""
         n = self.n
         # every element is written by the library
-        n_aln = np.empty(max(n, 1), np.int32)
-        rec_off = np.empty(max(n, 1), np.uint64)
-        pops = np.empty(max(n, 1), np.uint64) if want_pops else None
-        trace = np.empty(max(n, 1), np.uint64) if want_trace else None
-        ob = np.empty(max(n, 1), np.uint64) if want_bytes else None
+        def get(k, size, dt):
+            if out is None:
+                return np.empty(size, dt)
+            if k not in out:
+                raise GalnError(f"results: no '{k}' array in the output buffers")
+            return out[k]
+
+        n_aln = get("n_aln", max(n, 1), np.int32)
+        rec_off = get("rec_off", max(n, 1), np.uint64)
+        pops = get("pops", max(n, 1), np.uint64) if want_pops else None
+        trace = get("trace", max(n, 1), np.uint64) if want_trace else None
+        ob = get("bytes", max(n, 1), np.uint64) if want_bytes else None
+        for a_, dt in ((n_aln, np.int32), (rec_off, np.uint64), (pops, np.uint64), (trace, np.uint64), (ob, np.uint64)):
+            if a_ is not None and (a_.dtype != dt or a_.shape[0] < max(n, 1) or not a_.flags.c_contiguous):
+                raise GalnError("results: an output array has the wrong type or size")
         cap = rec_cap if rec_cap is not None else max(2 * n, 1)
+        if out is not None:
+            cap = out["rec"].shape[0]
         while True:
-            rec = np.empty((max(cap, 1), 3), np.uint64)
+            if out is not None and cap <= out["rec"].shape[0]:
+                rec = out["rec"]
+                if rec.dtype != np.uint64 or rec.ndim != 2 or rec.shape[1] != 3 or not rec.flags.c_contiguous:
+                    raise GalnError("results: the record array has the wrong type or shape")
+            else:
+                rec = np.empty((max(cap, 1), 3), np.uint64)
             O = Out()
             O.n_aln, O.rec_off, O.rec, O.rec_cap = _p(n_aln), _p(rec_off), _p(rec), cap
             O.n_pops = _p(pops) if pops is not None else None
@@ -427,6 +459,8 @@ class ReadSet:
             rc = hip_lib().galn_hip_readset_results(self.h, C.byref(O))
             if rc == E_RECCAP:
                 cap = int(O.n_rec)
+                if out is not None:
+                    out["rec"] = np.empty((max(cap, 1), 3), np.uint64)  # grown; kept for later calls
                 continue
             if rc != 0:
                 raise GalnError(f"readset_results failed ({rc}): {last_error()}")

@@ -135,6 +135,28 @@ def test_graph_replays_equal_eager_and_golden(name, key, golden_dir):
         rs.results()
 
 
+def test_results_into_reused_buffers(golden_dir):
+    """results(out=...) fills caller-owned arrays (reused, grown when the records outnumber
+    them) with what a fresh call returns."""
+    print("case results into reused buffers adna_mixed/adna_1", flush=True)
+    w, o, rs = readset_for("adna_mixed", "adna_1", golden_dir)
+    rs.search(nv.FORM_BATCHED)
+    fresh = rs.results(want_pops=True, want_bytes=True)
+    bufs = rs.result_buffers(want_pops=True, want_bytes=True, rec_cap=8)  # too small: grows
+    for rep in range(2):
+        rs.search(nv.FORM_BATCHED)
+        got = rs.results(want_pops=True, want_bytes=True, out=bufs)
+        np.testing.assert_array_equal(got.n_aln, fresh.n_aln)
+        np.testing.assert_array_equal(got.rec_off, fresh.rec_off)
+        np.testing.assert_array_equal(got.rec, fresh.rec)
+        np.testing.assert_array_equal(got.pops, fresh.pops)
+        np.testing.assert_array_equal(got.occ_bytes, fresh.occ_bytes)
+        assert bufs["rec"].shape[0] >= fresh.rec.shape[0]
+    assert fx.md5_bytes(nv.sai_bytes(o, got.n_aln, got.rec_off, got.rec)) == w["meta"]["runs"]["adna_1"]["sai_md5"]
+    with pytest.raises(nv.GalnError):
+        rs.results(want_trace=True, out=bufs)  # no trace array in these buffers
+
+
 def test_repeated_searches_identical(golden_dir):
     w, o, rs = readset_for("human_like", "adna_1", golden_dir)
     outs = []
