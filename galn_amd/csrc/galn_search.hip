@@ -268,9 +268,12 @@ __device__ __forceinline__ uint32_t lf_pair(const uint32_t *__restrict__ bwt, co
 
 __device__ __forceinline__ Ent mk_ent(uint64_t k, uint64_t l, int i, int ldp, int mm, int go, int ge, int ins, int st)
 {
+    // the fields first, so that children differing only in (k, l) share them
+    const uint64_t h0 = ((uint64_t)i << 38) | ((uint64_t)mm << 48) | ((uint64_t)go << 55) | ((uint64_t)st << 62);
+    const uint64_t h1 = ((uint64_t)ldp << 38) | ((uint64_t)ge << 48) | ((uint64_t)ins << 56);
     Ent e;
-    e.w0 = (k & KMASK) | ((uint64_t)i << 38) | ((uint64_t)mm << 48) | ((uint64_t)go << 55) | ((uint64_t)st << 62);
-    e.w1 = (l & KMASK) | ((uint64_t)ldp << 38) | ((uint64_t)ge << 48) | ((uint64_t)ins << 56);
+    e.w0 = h0 | (k & KMASK);
+    e.w1 = h1 | (l & KMASK);
     return e;
 }
 __device__ __forceinline__ uint64_t e_k(const Ent &e) { return e.w0 & KMASK; }
@@ -1588,6 +1591,13 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) k_
                                     }
                                 }
                                 const int tmp = (mode & MODE_LOGGAP) ? int_log2((uint32_t)(ge + go)) / 2 + 1 : go + ge;
+                                // the children's intervals per base
+                                uint64_t KK[4], LL[4];
+#pragma unroll
+                                for (int j = 0; j < 4; ++j) {
+                                    KK[j] = L2s[j] + ca[j] + 1;
+                                    LL[j] = L2s[j] + cb[j];
+                                }
                                 auto logp = [&](int s, const Ent &en) __attribute__((always_inline)) {
                                     if (kind != K_END) return;
                                     if (s < 0 || s >= n_stacks) {
@@ -1608,10 +1618,8 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) k_
                                         if (go < max_gapo) {
                                             const int sc = mm * s_mm + (go + 1) * s_gapo + ge * s_gape;
                                             logp(sc, mk_ent(k, l, i, i, mm, go + 1, ge, ins + 1, ST_I));
-                                            for (int j = 0; j < 4; ++j) {
-                                                const uint64_t kk = L2s[j] + ca[j] + 1, ll = L2s[j] + cb[j];
-                                                if (kk <= ll) logp(sc, mk_ent(kk, ll, i + 1, i + 1, mm, go + 1, ge, ins, ST_D));
-                                            }
+                                            for (int j = 0; j < 4; ++j)
+                                                if (KK[j] <= LL[j]) logp(sc, mk_ent(KK[j], LL[j], i + 1, i + 1, mm, go + 1, ge, ins, ST_D));
                                         }
                                     } else if (st == ST_I) {
                                         if (ge < max_gape)
@@ -1620,18 +1628,33 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) k_
                                     } else {
                                         if (ge < max_gape && (ge + go < max_diff || occ < (uint64_t)(int64_t)mdo)) {
                                             const int sc = mm * s_mm + go * s_gapo + (ge + 1) * s_gape;
-                                            for (int j = 0; j < 4; ++j) {
-                                                const uint64_t kk = L2s[j] + ca[j] + 1, ll = L2s[j] + cb[j];
-                                                if (kk <= ll) logp(sc, mk_ent(kk, ll, i + 1, i + 1, mm, go, ge + 1, ins, ST_D));
-                                            }
+                                            for (int j = 0; j < 4; ++j)
+                                                if (KK[j] <= LL[j]) logp(sc, mk_ent(KK[j], LL[j], i + 1, i + 1, mm, go, ge + 1, ins, ST_D));
                                         }
+                                    }
+                                }
+                                // the intervals rotated so that slot t holds base (qbase + t) & 3, the
+                                // order of the mismatch loop (the deletions above are done with them)
+                                uint64_t RK[4], RL[4];
+                                {
+                                    const bool r1 = (qbase & 1) != 0, r2 = (qbase & 2) != 0;
+                                    uint64_t TK[4], TL[4];
+#pragma unroll
+                                    for (int t = 0; t < 4; ++t) {
+                                        TK[t] = r1 ? KK[(t + 1) & 3] : KK[t];
+                                        TL[t] = r1 ? LL[(t + 1) & 3] : LL[t];
+                                    }
+#pragma unroll
+                                    for (int t = 0; t < 4; ++t) {
+                                        RK[t] = r2 ? TK[(t + 2) & 3] : TK[t];
+                                        RL[t] = r2 ? TL[(t + 2) & 3] : TL[t];
                                     }
                                 }
                                 if (allow_diff && allow_M) {
                                     for (int j = 1; j <= 4; ++j) {
-                                        const int c = (qbase + j) & 3;
+                                        // base (qbase + j) & 3: slot j & 3 of the rotated intervals
                                         const int is_mm = (j != 4 || qbase > 3);
-                                        const uint64_t kk = L2s[c] + ca[c] + 1, ll = L2s[c] + cb[c];
+                                        const uint64_t kk = RK[j & 3], ll = RL[j & 3];
                                         if (kk > ll) continue;
                                         if (is_mm) {
                                             logp((mm + 1) * s_mm + go * s_gapo + ge * s_gape,
@@ -1642,7 +1665,7 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) k_
                                         }
                                     }
                                 } else if (qbase < 4) {
-                                    const uint64_t kk = L2s[qbase] + ca[qbase] + 1, ll = L2s[qbase] + cb[qbase];
+                                    const uint64_t kk = RK[0], ll = RL[0];  // base qbase
                                     if (kk <= ll && kind == K_END) {
                                         child = mk_ent(kk, ll, i, 0, mm, go, ge, ins, ST_M);
                                         cont = true;
