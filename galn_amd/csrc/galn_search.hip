@@ -1016,6 +1016,17 @@ __global__ void __launch_bounds__(128) k_search(KArgs a)
                     // bit 0 insertion, bits 1-4 deletions (bases 0..3), bits 5-8 the bases
                     // qbase+1 .. qbase+4 (the last one is the exact child), bit 9 the exact child
                     // alone (when differences are not allowed here)
+                    // the children's intervals per base (constant indices: the counts come from
+                    // scalar loads, not per-lane loads of the kernel arguments)
+                    uint64_t KK[4], LL[4];
+#pragma unroll
+                    for (int j = 0; j < 4; ++j) {
+                        KK[j] = M.L2[j] + ca[j] + 1;
+                        LL[j] = M.L2[j] + cb[j];
+                    }
+                    auto pick = [](const uint64_t(&v)[4], int j) __attribute__((always_inline)) {
+                        return j == 0 ? v[0] : j == 1 ? v[1] : j == 2 ? v[2] : v[3];
+                    };
                     uint32_t mask = 0;
                     if (gap_ok) {
                         const bool open = st == ST_M && go < max_gapo;
@@ -1023,7 +1034,7 @@ __global__ void __launch_bounds__(128) k_search(KArgs a)
                         if (open || (st == ST_D && ge < max_gape && (ge + go < max_diff || occ < (uint64_t)(int64_t)mdo))) {
     #pragma unroll
                             for (int j = 0; j < 4; ++j)
-                                if (M.L2[j] + ca[j] + 1 <= M.L2[j] + cb[j]) mask |= 2u << j;
+                                if (KK[j] <= LL[j]) mask |= 2u << j;
                         }
                     }
                     if (allow_diff && allow_M) {
@@ -1064,12 +1075,12 @@ __global__ void __launch_bounds__(128) k_search(KArgs a)
                                 ch = mk_ent(k, l, i, i, mm, go2, ge2, ins + 1, ST_I);
                             } else if (bit <= 4) {
                                 const int j = bit - 1;
-                                const uint64_t kk = M.L2[j] + ca[j] + 1, ll = M.L2[j] + cb[j];
+                                const uint64_t kk = pick(KK, j), ll = pick(LL, j);
                                 const int go2 = st == ST_M ? go + 1 : go, ge2 = st == ST_M ? ge : ge + 1;
                                 ch = mk_ent(kk, ll, i + 1, i + 1, mm, go2, ge2, ins, ST_D);
                             } else {
                                 const int c = (qbase + bit - 4) & 3;
-                                const uint64_t kk = M.L2[c] + ca[c] + 1, ll = M.L2[c] + cb[c];
+                                const uint64_t kk = pick(KK, c), ll = pick(LL, c);
                                 ch = mk_ent(kk, ll, i, i, mm + 1, go, ge, ins, ST_M);
                             }
                             if (bit <= 4)
@@ -1082,7 +1093,7 @@ __global__ void __launch_bounds__(128) k_search(KArgs a)
                             CHK[slot_] = ch;
                         }
                         if (mx) {
-                            const uint64_t kk = M.L2[qbase] + ca[qbase] + 1, ll = M.L2[qbase] + cb[qbase];
+                            const uint64_t kk = pick(KK, qbase & 3), ll = pick(LL, qbase & 3);  // (qbase <= 3 here)
                             const Ent ch = mk_ent(kk, ll, i, 0, mm, go, ge, ins, ST_M);
                             if (a.fuse) {
                                 // pushed last with the parent's score: the next pop; kept in registers
