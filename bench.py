@@ -10,7 +10,7 @@ searched with `aln -l 1024 -n 0.01 -o 2`.
 A step is one search of the whole read set with the reads and the index resident in
 HBM, from the call into the library until every record is in host memory. The bench
 times K steps of the batched form (the reported value) and --plain-steps steps of the
-plain form, checks that both forms give identical results, runs the serving loop under
+plain form on the first --plain-reads reads, checks that both forms give identical records, runs the serving loop under
 Poisson arrivals at 25/50/75/90% of its peak (p50/p99 request latency), times the reference's CPU search
 (oracle/_ref, built from the reference's C sources) on a bounded sample of the same
 reads and checks the GPU records against it, and reports the dominant kernel's
@@ -223,7 +223,9 @@ def main() -> int:
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
-    ap.add_argument("--plain-steps", type=int, default=1, help="timed steps of the plain form (it is ~20x slower)")
+    ap.add_argument("--plain-steps", type=int, default=1, help="timed steps of the plain form")
+    ap.add_argument("--plain-reads", type=int, default=1_000_000,
+                    help="reads the plain form is timed on, the first ones of the workload (0: all; it is ~60x slower)")
     ap.add_argument("--serve-seconds", type=float, default=8.0, help="arrival time per offered load (0: no serving)")
     ap.add_argument("--serve-max-request", type=int, default=1_000_000)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -284,26 +286,38 @@ def main() -> int:
         rs.search(nv.FORM_BATCHED, BATCHED_FLAGS)
         rs.results(want_pops=False, out=bufs)
 
-    def timed(form: int, steps: int, out):
+    def timed(rset, n: int, form: int, steps: int, out):
         times, launches = [], []
         res = None
         for s in range(steps):
             torch.cuda.synchronize()
             t0 = time.perf_counter()
-            rs.search(form, BATCHED_FLAGS if form == nv.FORM_BATCHED else 0)
-            res = rs.results(want_pops=(s == steps - 1), want_bytes=(s == steps - 1), out=out)
+            rset.search(form, BATCHED_FLAGS if form == nv.FORM_BATCHED else 0)
+            res = rset.results(want_pops=(s == steps - 1), want_bytes=(s == steps - 1), out=out)
             torch.cuda.synchronize()
             times.append(time.perf_counter() - t0)
-            launches.append(rs.launches())
-            log(f"  form {form} step {s}: {times[-1] * 1e3:.1f} ms, {w.n / times[-1]:.3e} reads/s")
+            launches.append(rset.launches())
+            log(f"  form {form} step {s}: {times[-1] * 1e3:.1f} ms, {n / times[-1]:.3e} reads/s")
         return times, launches, res
 
     log(f"timing batched form, {args.steps} steps")
-    tb, lb, res_b = timed(nv.FORM_BATCHED, args.steps, bufs)
+    tb, lb, res_b = timed(rs, w.n, nv.FORM_BATCHED, args.steps, bufs)
+    n_plain = w.n if args.plain_reads <= 0 else min(args.plain_reads, w.n)
     if not args.no_plain:
-        log(f"timing plain form, {args.plain_steps} steps")
-        tp, lp, res_p = timed(nv.FORM_PLAIN, args.plain_steps, None)  # its own arrays: compared with res_b
-        same = np.array_equal(res_p.n_aln, res_b.n_aln) and np.array_equal(res_p.rec, res_b.rec)
+        # the plain form (~60x slower) on the first n_plain reads, a sample of the same generator;
+        # its records must equal the batched form's for those reads
+        lo_q = int(w.qoff[:n_plain].min())
+        hi_q = int((w.qoff[:n_plain] + w.lens[:n_plain].astype(np.uint64)).max())
+        rs_p = nv.ReadSet(index, w.query[lo_q:hi_q], w.qoff[:n_plain] - np.uint64(lo_q),
+                          w.lens[:n_plain].astype(np.int32), w.maxdiff[:n_plain], w.batch[:n_plain], w.bopts,
+                          w.nstacks)
+        log(f"timing plain form on the first {n_plain} reads, {args.plain_steps} steps")
+        tp, lp, res_p = timed(rs_p, n_plain, nv.FORM_PLAIN, args.plain_steps, None)
+        nb = n_plain
+        end_b = int(res_b.rec_off[nb - 1]) + max(int(res_b.n_aln[nb - 1]), 0)
+        same = (np.array_equal(res_p.n_aln, res_b.n_aln[:nb]) and np.array_equal(res_p.rec_off, res_b.rec_off[:nb])
+                and np.array_equal(res_p.rec, res_b.rec[:end_b]))
+        rs_p.close()
     else:
         tp, lp, same = [], [], None
     log(f"plain == batched: {same}")
@@ -348,9 +362,9 @@ def main() -> int:
                    "ref_bp": args.ref_bp, "reads": w.n, "aln_opts": " ".join(w.opts), "form": "batched",
                    "autotuned": tuned,
                    "index_bytes": ws},
-        "plain": {"value": w.n / (sum(tp) / len(tp)) if tp else None,
-                  "ms_per_step": 1e3 * sum(tp) / len(tp) if tp else None, "steps": len(tp)},
-        "batched_over_plain": (sum(tp) / len(tp)) / (sum(tb) / len(tb)) if tp else None,
+        "plain": {"value": n_plain / (sum(tp) / len(tp)) if tp else None,
+                  "ms_per_step": 1e3 * sum(tp) / len(tp) if tp else None, "steps": len(tp), "reads": n_plain},
+        "batched_over_plain": (value / (n_plain / (sum(tp) / len(tp)))) if tp else None,
         "parity": {"plain_eq_batched": same},
         "pops_per_read": {"mean": float(pops.mean()), "max": int(pops.max()), "log2_histogram": pops_hist},
         "kernel": {"dominant": dom, "launch_ms": k_ms, "launch_algo_bytes": k_bytes, "per_kernel": per,
