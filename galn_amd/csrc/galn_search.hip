@@ -1164,7 +1164,6 @@ constexpr int H_PUSH = 128;    // logged pushes per lane per round
 constexpr int H_CAP = 4096;    // pushes committed per round
 constexpr int H_KEEP = 64;     // granules a wave keeps between reads
 constexpr int H_PIECES = 48;    // segment pieces one round may add (three buckets)
-constexpr int H_GRP = 8;       // logged pushes a lane loads together when placing them
 constexpr uint32_t GENT = (uint32_t)GRAN * CH;  // entries per granule
 
 enum : int { K_END = 0, K_HIT = 1, K_TRUNC = 2, K_ESCORE = 3 };
@@ -2811,12 +2810,12 @@ int readset_search(galn_hip_readset *rs, int form, uint32_t flags, const galn_hi
                     hipGraph_t g = nullptr;
                     const hipError_t ec = hipStreamEndCapture(idx->stream, &g);
                     if (rc) {
-                        if (g) hipGraphDestroy(g);
+                        if (g) (void)hipGraphDestroy(g);
                         return rc;
                     }
                     if (ec != hipSuccess) return fail(GALN_HIP_E_DEVICE, "graph capture: %s", hipGetErrorString(ec));
                     const hipError_t ei = hipGraphInstantiate(out, g, nullptr, nullptr, 0);
-                    hipGraphDestroy(g);
+                    (void)hipGraphDestroy(g);
                     if (ei != hipSuccess) {
                         *out = nullptr;
                         return fail(GALN_HIP_E_DEVICE, "graph instantiate: %s", hipGetErrorString(ei));
@@ -2885,7 +2884,7 @@ int readset_reserve(galn_hip_readset *rs, int n, uint64_t qbytes, int nb)
     if (!grow_n && !grow_q && !grow_b) return 0;
     HIP_OK(hipStreamSynchronize(rs->idx->stream));
     auto regrow = [](void **p, size_t bytes) __attribute__((always_inline)) -> bool {
-        if (*p) hipFree(*p);
+        if (*p) (void)hipFree(*p);
         *p = nullptr;
         return hipMalloc(p, std::max<size_t>(bytes, 16)) == hipSuccess;
     };
@@ -3024,7 +3023,8 @@ int galn_hip_index_open(const uint32_t *bwt, const galn_bwt_meta_t *meta, int de
     if (hipStreamCreateWithFlags(&idx->stream, hipStreamNonBlocking) != hipSuccess ||
         hipMemsetAsync(idx->d_bwt, 0, alloc, idx->stream) != hipSuccess ||
         hipMemcpyAsync(idx->d_bwt, bwt, bytes, hipMemcpyHostToDevice, idx->stream) != hipSuccess) {
-        hipFree(idx->d_bwt);
+        if (idx->stream) (void)hipStreamDestroy(idx->stream);
+        (void)hipFree(idx->d_bwt);
         delete idx;
         return fail(GALN_HIP_E_DEVICE, "index upload failed");
     }
@@ -3033,7 +3033,8 @@ int galn_hip_index_open(const uint32_t *bwt, const galn_bwt_meta_t *meta, int de
         hipLaunchKernelGGL(k_planes, dim3((unsigned)((n_blocks + 255) / 256)), dim3(256), 0, idx->stream, idx->d_bwt,
                            n_blocks);
     if (hipGetLastError() != hipSuccess || hipStreamSynchronize(idx->stream) != hipSuccess) {
-        hipFree(idx->d_bwt);
+        (void)hipStreamDestroy(idx->stream);
+        (void)hipFree(idx->d_bwt);
         delete idx;
         return fail(GALN_HIP_E_DEVICE, "index upload failed");
     }
@@ -3047,13 +3048,13 @@ int galn_hip_index_close(galn_hip_index *idx)
     if (idx->n_readsets.load() > 0)
         return fail(GALN_HIP_E_BUSY, "index_close: %d read sets (or servers) of this index are still open",
                     idx->n_readsets.load());
-    hipSetDevice(idx->device);
-    if (idx->stream) hipStreamSynchronize(idx->stream);
-    hipFree(idx->d_bwt);
+    (void)hipSetDevice(idx->device);
+    if (idx->stream) (void)hipStreamSynchronize(idx->stream);
+    (void)hipFree(idx->d_bwt);
     void *pool[] = {idx->d_pool_chk, idx->d_pool_link, idx->d_gnext, idx->d_gbump, idx->d_ghead};
     for (void *p : pool)
-        if (p) hipFree(p);
-    if (idx->stream) hipStreamDestroy(idx->stream);
+        if (p) (void)hipFree(p);
+    if (idx->stream) (void)hipStreamDestroy(idx->stream);
     delete idx;
     return 0;
 }
@@ -3063,19 +3064,19 @@ int galn_hip_index_device(const galn_hip_index *idx) { return idx ? idx->device 
 int galn_hip_readset_free(galn_hip_readset *rs)
 {
     if (!rs) return 0;
-    hipSetDevice(rs->idx->device);
-    hipStreamSynchronize(rs->idx->stream);
+    (void)hipSetDevice(rs->idx->device);
+    (void)hipStreamSynchronize(rs->idx->stream);
     void *ptrs[] = {rs->d_query, rs->d_qoff, rs->d_qlen, rs->d_maxdiff, rs->d_batch, rs->d_nstacks, rs->d_bopts,
                     rs->d_sorted, rs->d_out_n, rs->d_out_off, rs->d_pops, rs->d_trace, rs->d_bytes, rs->d_rec, rs->d_top,
                     rs->d_fail[0], rs->d_fail[1], rs->d_ctr, rs->d_scratch, rs->d_algo, rs->d_heavy, rs->d_hscratch,
                     rs->d_stats, rs->d_incl, rs->d_roff, rs->d_stage, rs->d_scan_tmp, rs->d_hand};
     for (void *p : ptrs)
-        if (p) hipFree(p);
+        if (p) (void)hipFree(p);
     for (auto &e : rs->ev)
-        if (e) hipEventDestroy(e);
-    for (auto &e : rs->lev) hipEventDestroy(e);
-    if (rs->gexec) hipGraphExecDestroy(rs->gexec);
-    if (rs->gexec2) hipGraphExecDestroy(rs->gexec2);
+        if (e) (void)hipEventDestroy(e);
+    for (auto &e : rs->lev) (void)hipEventDestroy(e);
+    if (rs->gexec) (void)hipGraphExecDestroy(rs->gexec);
+    if (rs->gexec2) (void)hipGraphExecDestroy(rs->gexec2);
     rs->idx->n_readsets.fetch_sub(1);
     delete rs;
     return 0;
