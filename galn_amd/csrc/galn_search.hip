@@ -1162,6 +1162,7 @@ constexpr int HW = 64;         // lanes, one wave per workgroup
 constexpr int H_STEPS = 32;    // chain steps per lane per round
 constexpr int H_PUSH = 128;    // logged pushes per lane per round
 constexpr int H_CAP = 4096;    // pushes committed per round
+static_assert(H_CAP < 65536, "a round's bucket totals are summed in 16-bit fields");
 constexpr int H_KEEP = 64;     // granules a wave keeps between reads
 constexpr int H_PIECES = 48;    // segment pieces one round may add (three buckets)
 constexpr uint32_t GENT = (uint32_t)GRAN * CH;  // entries per granule
@@ -1754,8 +1755,11 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) k_
                     ncp += SINFO[t] & 0xf;
                 }
             }
-            const unsigned long long sum_pops = wave_sum_u64(cj), sum_by = wave_sum_u64(by);
-            const int sum_ne = wave_sum_i32(cj > 0 ? ncp - 1 : 0);
+            // pops and entry changes in one sum: pops in bits 0-31, entry change + 1 (>= 0) above
+            const unsigned long long pe_sum =
+                wave_sum_u64((unsigned long long)cj | (unsigned long long)(cj > 0 ? ncp : 1) << 32);
+            const unsigned long long sum_pops = pe_sum & 0xffffffffull, sum_by = wave_sum_u64(by);
+            const int sum_ne = (int)(pe_sum >> 32) - HW;
             if (TRACE) {
                 lcommit[lane] = cj;
                 lnstart[lane] = nstart;
@@ -1805,7 +1809,10 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) k_
             const uint32_t cc = ncp > 0 ? ccnt : 0u;
             const int n0 = (int)(cc & 0xff), n1 = (int)((cc >> 8) & 0xff), n2 = (int)((cc >> 16) & 0xff);
             const bool bad = (cc >> 24) != 0 || (ncp != 0 && ncp != np);
-            const int T0 = wave_sum_i32(n0), T1 = wave_sum_i32(n1), T2 = wave_sum_i32(n2);
+            // the three totals in one sum (16-bit fields: a round commits at most H_CAP pushes)
+            const unsigned long long tsum =
+                wave_sum_u64((unsigned long long)n0 | (unsigned long long)n1 << 16 | (unsigned long long)n2 << 32);
+            const int T0 = (int)(tsum & 0xffff), T1 = (int)((tsum >> 16) & 0xffff), T2 = (int)((tsum >> 32) & 0xffff);
             const bool badw = __ballot(bad) != 0;
             if (lane == 0) {
                 // pieces of the wave's current granule for each bucket that receives pushes; a
