@@ -258,6 +258,54 @@ __device__ __forceinline__ uint32_t lf_pair(const uint32_t *__restrict__ bwt, co
     return (aneg || same) ? 64u : 128u;
 }
 
+// Occ of base c alone at positions a (may be -1) and b, for the steps that follow one symbol (an
+// exact tail, a width): one popcount per plane word, the planes flipped so that base c reads as
+// ones. Same block loads and byte count as lf_pair.
+__device__ __forceinline__ uint64_t block_occ1(const uint4 (&b)[4], int r, int c)
+{
+    const uint32_t hp[4] = {b[2].x, b[2].y, b[2].z, b[2].w};
+    const uint32_t lp[4] = {b[3].x, b[3].y, b[3].z, b[3].w};
+    const uint32_t hx = (c & 2) ? 0u : ~0u, lx = (c & 1) ? 0u : ~0u;
+    uint32_t n = 0;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        const uint32_t t = (uint32_t)min(max(r + 1 - 32 * j, 0), 32);
+        const uint32_t valid = (uint32_t)(0xffffffff00000000ull >> t);
+        n += __builtin_popcount((hp[j] ^ hx) & (lp[j] ^ lx) & valid);
+    }
+    // base c's count at the block start, by masks (a select between the block's words would be
+    // turned into an indexed load of the block, which then lives in scratch memory)
+    const uint32_t m0 = 0u - (uint32_t)(c == 0), m1 = 0u - (uint32_t)(c == 1), m2 = 0u - (uint32_t)(c == 2),
+                   m3 = 0u - (uint32_t)(c == 3);
+    const uint32_t lo = (b[0].x & m0) | (b[0].z & m1) | (b[1].x & m2) | (b[1].z & m3);
+    const uint32_t hi = (b[0].y & m0) | (b[0].w & m1) | (b[1].y & m2) | (b[1].w & m3);
+    return ((uint64_t)hi << 32 | lo) + n;
+}
+
+__device__ __forceinline__ uint32_t lf_one(const uint32_t *__restrict__ bwt, const DMeta &m, uint64_t a, uint64_t b,
+                                           int c, uint64_t &oa, uint64_t &ob)
+{
+    const uint64_t bp = b - (b >= m.primary);
+    const uint64_t bb = bp >> 7;
+    const bool aneg = a == ~0ull;
+    const uint64_t ap = a - (a >= m.primary);
+    const uint64_t ab = ap >> 7;
+    const bool same = !aneg && ab == bb;
+    uint4 B[4], A[4];
+    load_block(bwt, bb, B);
+    if (!aneg && !same) load_block(bwt, ab, A);
+    ob = block_occ1(B, (int)(bp & 127), c);
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+        A[t].x = same ? B[t].x : A[t].x;
+        A[t].y = same ? B[t].y : A[t].y;
+        A[t].z = same ? B[t].z : A[t].z;
+        A[t].w = same ? B[t].w : A[t].w;
+    }
+    oa = aneg ? 0 : block_occ1(A, (int)(ap & 127), c);
+    return (aneg || same) ? 64u : 128u;
+}
+
 // ---------------------------------------------------------------------------
 // Packed stack entry (16 bytes); the score is the bucket.
 //   w0 = k[0:38) | i[38:48) | n_mm[48:55) | n_gapo[55:62) | state[62:64)
@@ -1243,11 +1291,6 @@ __device__ __forceinline__ int wave_excl_scan(int v)
     return x - v;
 }
 
-__device__ __forceinline__ int wave_sum_i32(int v)
-{
-    for (int d = HW / 2; d > 0; d >>= 1) v += __shfl_xor(v, d, HW);
-    return v;
-}
 
 __device__ __forceinline__ unsigned long long wave_sum_u64(unsigned long long v)
 {
@@ -1265,16 +1308,14 @@ __device__ __forceinline__ int wave_max_i32(int v)
 __device__ uint64_t cal_width_one(const uint32_t *bwt, const DMeta &M, int len, const uint8_t *str, uint64_t *w)
 {
     const L2Sel L2s(M);
-    uint64_t k = 0, l = M.seq_len, ca[4], cb[4], bytes = 0;
+    uint64_t k = 0, l = M.seq_len, bytes = 0;
     int bid = 0;
     for (int i = 0; i < len; ++i) {
         const int c0 = str[i];
         const int c = c0 > 3 ? 4 : 3 - c0;
         if (c < 4) {
-            bytes += lf_pair(bwt, M, k - 1, l, ca, cb);
-            // (selects: a count array indexed by the symbol would live in scratch memory)
-            const uint64_t oa = c == 0 ? ca[0] : c == 1 ? ca[1] : c == 2 ? ca[2] : ca[3];
-            const uint64_t ob = c == 0 ? cb[0] : c == 1 ? cb[1] : c == 2 ? cb[2] : cb[3];
+            uint64_t oa, ob;
+            bytes += lf_one(bwt, M, k - 1, l, c, oa, ob);
             k = L2s[c] + oa + 1;
             l = L2s[c] + ob;
         }
@@ -1568,9 +1609,10 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) k_
                                             ok = false;
                                             break;
                                         }
-                                        sbytes += lf_pair(a.bwt, M, k - 1, l, ca, cb);
-                                        k = L2s[c] + ca[c] + 1;
-                                        l = L2s[c] + cb[c];
+                                        uint64_t oa, ob;
+                                        sbytes += lf_one(a.bwt, M, k - 1, l, c, oa, ob);
+                                        k = L2s[c] + oa + 1;
+                                        l = L2s[c] + ob;
                                         ok = k <= l;
                                     }
                                     if (ok && (int)(uint32_t)(l - k + 1) != 0) {
