@@ -31,6 +31,7 @@
 // the capacities chosen (the reference's contract, src/bwtgap_hd.h:10-22).
 
 #include <hip/hip_runtime.h>
+#include <cstdlib>
 #include <cstring>
 #include <rocprim/device/device_scan.hpp>
 #include <rocprim/iterator/transform_iterator.hpp>
@@ -1254,6 +1255,7 @@ struct HArgs {
     int32_t buckets;  // LDS buckets: the largest n_stacks of the launch
     int32_t wlen;     // LDS widths: longest read + 1
     int32_t swlen;    // LDS seed widths
+    int32_t push_cap; // pushes committed per round, H_PUSH..H_CAP (H_CAP unless a test lowers it)
     unsigned long long *stats;  // [8] engine counters (rounds, chains, steps, cycles)
 };
 
@@ -1754,8 +1756,8 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) k_
             const int nstart = F.n_entries + wave_excl_scan(inJ ? np - 1 + (kind == K_TRUNC ? 1 : 0) : 0);
             // the push cap falls in chain jcap, which waits for the next round with the chains
             // after it (chain 0 always fits: a chain logs at most H_PUSH pushes)
-            const uint64_t capm = __ballot(inJ && pex + np > H_CAP);
-            const int jcap = capm ? __builtin_ctzll(capm) : J + 1;
+            const uint64_t capm = __ballot(inJ && pex + np > a.push_cap);
+            const int jcap = capm ? max(1, __builtin_ctzll(capm)) : J + 1;
             const bool cand = inJ && lane < jcap;
             int c = 0, cut = -1;
             if (cand) {
@@ -2207,6 +2209,7 @@ struct GraphKey {
     const void *hand;
     int32_t hand_cap;
     uint32_t hand_stride;
+    int32_t push_cap;
 };
 
 struct galn_hip_readset {
@@ -2595,6 +2598,17 @@ int run_tier(galn_hip_readset *rs, uint32_t flags, const Tier &t, const int32_t 
     return 0;
 }
 
+// Pushes the wave engine commits per round: H_CAP, or a lower value (at least H_PUSH, so that a
+// round's first chain always fits) from GALN_HIP_TEST_PUSH_CAP, which the tests set to make the
+// cap cut rounds short and check that the results do not change.
+int heavy_push_cap()
+{
+    const char *v = getenv("GALN_HIP_TEST_PUSH_CAP");
+    if (!v || !*v) return H_CAP;
+    const long c = strtol(v, nullptr, 10);
+    return (int)std::min<long>(H_CAP, std::max<long>(H_PUSH, c));
+}
+
 // The wave-per-read engine over list[0..n): one wave per read, `waves` resident waves
 // claiming reads from a device counter; failures land in fail buffer `fo`.
 int run_heavy(galn_hip_readset *rs, uint32_t flags, const int32_t *list, int n, int rec_cap, int fo, int tier_id,
@@ -2656,6 +2670,7 @@ int run_heavy(galn_hip_readset *rs, uint32_t flags, const int32_t *list, int n, 
     a.fail_kind = rs->d_ctr + 4;
     a.buckets = buckets_of(rs);
     a.wlen = wlen;
+    a.push_cap = heavy_push_cap();
     a.swlen = swlen;
     a.stats = (flags & GALN_HIP_STATS) ? rs->d_stats : nullptr;
     FILL(rs->d_ctr + 2, 0, sizeof(int32_t));
@@ -2832,6 +2847,7 @@ int readset_search(galn_hip_readset *rs, int form, uint32_t flags, const galn_hi
             memset(&key, 0, sizeof key);  // padding included: keys are compared bytewise
             key.flags = flags, key.block = block, key.slots = slots, key.hwaves = hwaves, key.n = rs->n;
             key.budget = budget, key.rec_cap = (uint32_t)t0.rec_cap, key.keep = (uint32_t)t0.keep, key.list = list;
+            key.push_cap = heavy_push_cap();
             if ((r = prepare_tier(rs, t0, slots)) || (budget && (r = prepare_heavy(rs, hwaves, 256)))) return r;
             while (rs->lev.size() < 4) {
                 hipEvent_t e;

@@ -72,6 +72,29 @@ def test_golden_sai_pops_trace(name, key, form, golden_dir):
     assert fx.md5_bytes(nv.sai_bytes(o, res.n_aln, res.rec_off, res.rec)) == w["meta"]["runs"][key]["sai_md5"]
 
 
+@pytest.mark.parametrize("name,key,must_cut", [("deep_repeats", "deep_1", True), ("adna_mixed", "adna_1", True),
+                                               ("options", "max_entries_m300_1", False)])
+def test_push_cap_cuts_rounds_without_changing_results(name, key, must_cut, golden_dir, monkeypatch):
+    """With the wave engine's per-round push cap lowered to 128 (every read forced through the
+    engine), rounds end at the cap and the deferred chains run in the next round: records, pops
+    and traces stay the reference's."""
+    print(f"case push cap {name}/{key}", flush=True)
+    w, o, rs = readset_for(name, key, golden_dir)
+    t = tune(heavy_pops=1)
+    rs.search(nv.FORM_BATCHED, nv.GALN_HIP_TRACE | nv.GALN_HIP_STATS, t)
+    base_rounds = rs.engine_stats()["rounds"]
+    monkeypatch.setenv("GALN_HIP_TEST_PUSH_CAP", "128")
+    rs.search(nv.FORM_BATCHED, nv.GALN_HIP_TRACE | nv.GALN_HIP_STATS, t)
+    rounds = rs.engine_stats()["rounds"]
+    res = rs.results(want_pops=True, want_trace=True)
+    exp = w["expected"]
+    np.testing.assert_array_equal(res.n_aln, exp[key + "_n_aln"])
+    np.testing.assert_array_equal(res.pops, exp[key + "_pops"])
+    np.testing.assert_array_equal(res.trace, exp[key + "_trace"])
+    assert fx.md5_bytes(nv.sai_bytes(o, res.n_aln, res.rec_off, res.rec)) == w["meta"]["runs"][key]["sai_md5"]
+    assert rounds > base_rounds if must_cut else rounds >= base_rounds
+
+
 @pytest.mark.parametrize("form", ["plain", "batched"])
 @pytest.mark.parametrize("name,key", [("plain_small", "default_1"), ("adna_mixed", "adna_1"), ("deep_repeats", "deep_2"),
                                       ("options", "max_entries_m300_1"), ("fastq_edge", "adna_1"),
