@@ -1863,11 +1863,10 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) k_
                 // bucket's new piece links to its previous top segment
                 int np_ = 0;
                 bool oom = false;
-                const int tgs[3] = {tg0, tg1, tg2};
-                const int tot[3] = {T0, T1, T2};
                 for (int x = 0; x < 3 && !oom && !badw; ++x) {
-                    const int sb = tgs[x];
-                    uint32_t rem = (uint32_t)tot[x];
+                    // (selects: arrays indexed by x would live in scratch memory)
+                    const int sb = x == 0 ? tg0 : x == 1 ? tg1 : tg2;
+                    uint32_t rem = (uint32_t)(x == 0 ? T0 : x == 1 ? T1 : T2);
                     if (!rem) continue;
                     const bool bit = (F.bm[sb >> 6] >> (sb & 63)) & 1ull;
                     uint32_t prev_base = bit ? seg_base[sb] : NIL, prev_size = bit ? seg_cnt[sb] : 0u, rk = 0;
