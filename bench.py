@@ -61,6 +61,33 @@ def parse_sai(path: str, n: int):
     return n_aln, recs
 
 
+def host_cpu_info() -> dict:
+    """The host's CPUs: logical CPUs, those this process may run on, model name and sockets."""
+    model, sockets = None, set()
+    try:
+        for ln in open("/proc/cpuinfo"):
+            k, _, v = ln.partition(":")
+            k = k.strip()
+            if k == "model name" and model is None:
+                model = v.strip()
+            elif k == "physical id":
+                sockets.add(v.strip())
+    except OSError:
+        pass
+    return {"nproc": os.cpu_count(), "affinity": len(os.sched_getaffinity(0)), "model": model,
+            "sockets": len(sockets) or None}
+
+
+def cpu_threads() -> int:
+    """Every CPU this process may run on, within the CPU share its host grants (OMP_NUM_THREADS,
+    when the host sets it)."""
+    n = len(os.sched_getaffinity(0))
+    share = os.environ.get("OMP_NUM_THREADS", "")
+    if share.isdigit() and int(share) > 0:
+        n = min(n, int(share))
+    return max(1, n)
+
+
 def cpu_baseline(w, res, sample: int, threads: int, workdir: str) -> dict:
     """The reference's CPU search on the first `sample` reads, and its agreement with the GPU."""
     ref_bin = os.path.join(ROOT, "oracle", "_ref", "galn_ref")
@@ -98,7 +125,7 @@ def cpu_baseline(w, res, sample: int, threads: int, workdir: str) -> dict:
                 break
     return {"value": sample / secs, "unit": "reads/s", "cores": threads, "kind": kind, "seconds": secs,
             "sample": f"first {sample} reads of the same workload, {threads} host threads, interleaved search",
-            "records_equal_gpu": agree}
+            "host": host_cpu_info(), "records_equal_gpu": agree}
 
 
 TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "k_search_k_heavy_adna10M_traffic.md")
@@ -234,8 +261,8 @@ def main() -> int:
     ap.add_argument("--workdir", default="")
     args = ap.parse_args()
     if args.gpus != 1:
-        sys.stderr.write("bench.py: this build measures one GPU only (the search is per-GPU; see DESIGN.md "
-                         "for how reads would shard); run with --gpus 1\n")
+        sys.stderr.write("bench.py: measures one GPU only (reads shard across GPUs by read, each GPU holding "
+                         "the whole index, with no exchange during the search); run with --gpus 1\n")
         return 2
 
     import torch
@@ -248,7 +275,7 @@ def main() -> int:
         return 3
     workdir = args.workdir or tempfile.mkdtemp(prefix="galn_bench_")
     os.makedirs(workdir, exist_ok=True)
-    threads = args.cpu_threads or min(os.cpu_count() or 1, 16)
+    threads = args.cpu_threads or cpu_threads()
 
     log(f"building workload: {args.ref_bp} bp reference, {args.reads} reads (index on the GPU)")
     t = time.time()
@@ -371,7 +398,12 @@ def main() -> int:
                    "all_launches_ms": sum(l["ms"] for l in lb[-1]), "launches": len(lb[-1]),
                    "algo_bytes_total": total_algo},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic},
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "achieved_is": "algorithmic index bytes (64 per distinct 128-symbol block per occurrence "
+                                    "query) of the dominant kernel's launch / its duration",
+                     "traffic_source": (f"profiled: {os.path.relpath(TRAFFIC_PROFILE, ROOT)} (rocprofv3 --pmc "
+                                        "FETCH_SIZE, WRITE_SIZE of the same workload and tuning)")
+                     if traffic is not None else None},
         "ceiling": {"kind": "pointer chase, 64-byte random lines", "ws_bytes": ws, "GB/s": ceiling_gbs,
                     "frac": achieved / ceiling_gbs if ceiling_gbs else None},
     }

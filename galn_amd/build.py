@@ -1,7 +1,7 @@
 """Builds the native parts in-tree:
 
   galn_amd/lib/libgaln_hip.so   HIP search kernels + C ABI + serving loop (hipcc, gfx950)
-  galn_amd/lib/libgaln_host.so  host side of `galn aln` (g++), links libgaln_hip
+  galn_amd/lib/libgaln_host.so  host side of `galn aln` and the CPU search engine (g++), links libgaln_hip
   galn_amd/bin/galn             the `galn` executable
   oracle/liboracle.so           the CPU oracle (test infrastructure), and oracle/_ref/ when
                                 the reference tree is present (see oracle/Makefile)
@@ -56,8 +56,8 @@ def build_host() -> tuple[str, str]:
     os.makedirs(LIB, exist_ok=True)
     os.makedirs(BIN, exist_ok=True)
     lib = os.path.join(LIB, "libgaln_host.so")
-    _run(["g++", *HOST_FLAGS, "-shared", f"-I{INC}", "-o", lib + ".tmp",
-          os.path.join(CSRC, "host", "galn_host.cpp"), f"-L{LIB}", "-lgaln_hip",
+    _run(["g++", *HOST_FLAGS, "-O3", "-shared", f"-I{INC}", "-o", lib + ".tmp",
+          os.path.join(CSRC, "host", "galn_host.cpp"), os.path.join(CSRC, "host", "galn_cpu.cpp"), f"-L{LIB}", "-lgaln_hip",
           "-Wl,-rpath,$ORIGIN", "-lz", "-lpthread"])
     os.replace(lib + ".tmp", lib)
     exe = os.path.join(BIN, "galn")

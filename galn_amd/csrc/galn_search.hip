@@ -2194,6 +2194,7 @@ struct galn_hip_index {
     unsigned long long *d_ghead = nullptr;
     uint32_t n_gran = 0;
     std::atomic<int> n_readsets{0};  // read sets not yet freed (they use the stream and pool)
+    std::atomic<int> n_servers{0};   // open servers: the index then belongs to their workers
 };
 
 struct GraphKey {
@@ -2319,6 +2320,15 @@ int ensure_scratch(galn_hip_readset *rs, size_t bytes)
     return 0;
 }
 
+// Granules the stack pool may hold at most: no limit, or the value of GALN_HIP_TEST_POOL_GRANULES,
+// which the tests set to exhaust the pool and drive the re-runs with fewer lanes.
+uint64_t pool_test_cap()
+{
+    const char *v = getenv("GALN_HIP_TEST_POOL_GRANULES");
+    if (!v || !*v) return ~0ull;
+    return (uint64_t)std::max(1L, strtol(v, nullptr, 10));
+}
+
 // The stack pool: `want` granules of 64 chunks (16 KB each), within half the free memory.
 int ensure_pool(galn_hip_index *idx, uint64_t want)
 {
@@ -2329,22 +2339,43 @@ int ensure_pool(galn_hip_index *idx, uint64_t want)
     uint64_t cap = (uint64_t)((freeb + have * (gbytes + GRAN * 4 + 4)) / 2) / (gbytes + GRAN * 4 + 4);
     cap = std::min<uint64_t>(cap, ((uint64_t)48 << 30) / gbytes);
     cap = std::min<uint64_t>(cap, MAX_POOL_CHUNKS / GRAN);
+    cap = std::min<uint64_t>(cap, pool_test_cap());
     want = std::min(std::max<uint64_t>(want, 1024), cap);
     if (have >= want) return 0;
-    if (idx->d_pool_chk) {
-        HIP_OK(hipFree(idx->d_pool_chk));
-        HIP_OK(hipFree(idx->d_pool_link));
-        HIP_OK(hipFree(idx->d_gnext));
-        idx->d_pool_chk = nullptr;
-        idx->n_gran = 0;
-    }
     if (!idx->d_ghead) {
-        HIP_OK(hipMalloc(&idx->d_ghead, NSHARD * 8));
-        HIP_OK(hipMalloc(&idx->d_gbump, 4));
+        unsigned long long *head = nullptr;
+        uint32_t *bump = nullptr;
+        HIP_OK(hipMalloc(&head, NSHARD * 8));
+        if (hipMalloc(&bump, 4) != hipSuccess) {
+            (void)hipFree(head);
+            return fail(GALN_HIP_E_NOMEM, "stack pool: allocation of its heads failed");
+        }
+        idx->d_ghead = head;
+        idx->d_gbump = bump;
     }
-    HIP_OK(hipMalloc(&idx->d_pool_chk, want * gbytes));
-    HIP_OK(hipMalloc(&idx->d_pool_link, want * GRAN * 4));
-    HIP_OK(hipMalloc(&idx->d_gnext, want * 4));
+    // the old pool goes first (it is not copied: no search is running), then all three new
+    // arrays are allocated before any of them is installed, so a failure leaves an empty pool
+    // (null pointers, no granules) and nothing freed twice
+    void *old[] = {idx->d_pool_chk, idx->d_pool_link, idx->d_gnext};
+    idx->d_pool_chk = nullptr;
+    idx->d_pool_link = nullptr;
+    idx->d_gnext = nullptr;
+    idx->n_gran = 0;
+    for (void *p : old)
+        if (p) HIP_OK(hipFree(p));
+    Ent *chk = nullptr;
+    uint32_t *link = nullptr, *gnext = nullptr;
+    if (hipMalloc(&chk, want * gbytes) != hipSuccess || hipMalloc(&link, want * GRAN * 4) != hipSuccess ||
+        hipMalloc(&gnext, want * 4) != hipSuccess) {
+        (void)hipGetLastError();
+        for (void *p : {(void *)chk, (void *)link, (void *)gnext})
+            if (p) (void)hipFree(p);
+        return fail(GALN_HIP_E_NOMEM, "stack pool of %llu granules does not fit in device memory",
+                    (unsigned long long)want);
+    }
+    idx->d_pool_chk = chk;
+    idx->d_pool_link = link;
+    idx->d_gnext = gnext;
     idx->n_gran = (uint32_t)want;
     return 0;
 }
@@ -3052,6 +3083,20 @@ const char *galn_hip_last_error(void) { return g_err; }
 // the count of handles that keep an index open.
 __attribute__((visibility("hidden"))) void galn_hip_set_error(const char *msg) { snprintf(g_err, sizeof(g_err), "%s", msg); }
 __attribute__((visibility("hidden"))) void galn_hip_index_hold(galn_hip_index *idx, int delta) { idx->n_readsets += delta; }
+// A server's worker threads mark themselves; while a server is open on an index, only they may
+// search it (its stream and stack pool are theirs).
+thread_local bool tl_server_worker = false;
+__attribute__((visibility("hidden"))) void galn_hip_server_worker_mark() { tl_server_worker = true; }
+__attribute__((visibility("hidden"))) void galn_hip_index_serve(galn_hip_index *idx, int delta) { idx->n_servers += delta; }
+
+namespace {
+int busy_check(const galn_hip_index *idx, const char *what)
+{
+    if (idx && idx->n_servers.load() > 0 && !tl_server_worker)
+        return fail(GALN_HIP_E_BUSY, "%s: a server holds this index until galn_hip_server_close", what);
+    return 0;
+}
+}  // namespace
 
 extern "C" {
 
@@ -3150,8 +3195,9 @@ int galn_hip_readset_upload(galn_hip_index *idx, const galn_hip_reads *reads, ga
 {
     if (!idx || !out) return fail(GALN_HIP_E_ARG, "readset_upload: null argument");
     *out = nullptr;
-    int r = check_reads(reads);
+    int r = busy_check(idx, "readset_upload");
     if (r) return r;
+    if ((r = check_reads(reads))) return r;
     HIP_OK(hipSetDevice(idx->device));
     galn_hip_readset *rs = new galn_hip_readset;
     rs->idx = idx;
@@ -3179,8 +3225,9 @@ int galn_hip_readset_upload(galn_hip_index *idx, const galn_hip_reads *reads, ga
 int galn_hip_readset_reload(galn_hip_readset *rs, const galn_hip_reads *reads)
 {
     if (!rs) return fail(GALN_HIP_E_ARG, "readset_reload: null handle");
-    int r = check_reads(reads);
+    int r = busy_check(rs->idx, "readset_reload");
     if (r) return r;
+    if ((r = check_reads(reads))) return r;
     HIP_OK(hipSetDevice(rs->idx->device));
     return readset_fill(rs, reads);
 }
@@ -3189,6 +3236,8 @@ int galn_hip_readset_search(galn_hip_index *idx, galn_hip_readset *rs, int form,
                             const galn_hip_tune_t *tune)
 {
     if (!idx || !rs || rs->idx != idx) return fail(GALN_HIP_E_ARG, "readset_search: bad handle");
+    const int b = busy_check(idx, "readset_search");
+    if (b) return b;
     if (rs->n == 0) return 0;
     return readset_search(rs, form, flags, tune);
 }
@@ -3196,6 +3245,8 @@ int galn_hip_readset_search(galn_hip_index *idx, galn_hip_readset *rs, int form,
 int galn_hip_readset_autotune(galn_hip_index *idx, galn_hip_readset *rs, uint32_t flags, galn_hip_tune_t *out)
 {
     if (!idx || !rs || rs->idx != idx) return fail(GALN_HIP_E_ARG, "readset_autotune: bad handle");
+    const int b = busy_check(idx, "readset_autotune");
+    if (b) return b;
     rs->has_tuned = false;
     galn_hip_tune_t best{};
     best.sort_by_cost = 1;
@@ -3242,6 +3293,8 @@ int galn_hip_readset_results(galn_hip_readset *rs, galn_hip_out *out)
     if (!rs || !out || (rs->n > 0 && (!out->n_aln || !out->rec_off)))
         return fail(GALN_HIP_E_ARG, "readset_results: null argument");
     galn_hip_index *idx = rs->idx;
+    const int b = busy_check(idx, "readset_results");
+    if (b) return b;
     HIP_OK(hipSetDevice(idx->device));
     const int n = rs->n;
     if (n == 0) {

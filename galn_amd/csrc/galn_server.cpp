@@ -13,6 +13,7 @@
 #include <condition_variable>
 #include <cstdarg>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <deque>
 #include <map>
@@ -63,9 +64,13 @@ uint64_t footprint_of(const galn_hip_reads *r, uint64_t qbytes)
 
 }  // namespace
 
-// galn_search.hip: this thread's galn_hip_last_error text; the open-handle count of an index
+// galn_search.hip: this thread's galn_hip_last_error text; the open-handle count of an index;
+// the count of servers holding an index, and the mark of their worker threads (only those may
+// search an index that a server holds)
 void galn_hip_set_error(const char *msg);
 void galn_hip_index_hold(galn_hip_index *idx, int delta);
+void galn_hip_index_serve(galn_hip_index *idx, int delta);
+void galn_hip_server_worker_mark();
 
 namespace {
 
@@ -95,6 +100,10 @@ struct galn_hip_server {
     std::thread worker;
     galn_hip_readset *rs = nullptr;
     double stats[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    // GALN_HIP_TEST_SERVER_FAIL=n: the n-th launch (from 1) fails on the host before it reaches
+    // the device, so that tests can drive the error path of a request spread over several launches
+    long fail_at = 0;
+    long n_launch = 0;
 };
 
 namespace {
@@ -125,6 +134,7 @@ void schedule(galn_hip_server *S, std::vector<Piece> &pieces)
 
 void worker_main(galn_hip_server *S)
 {
+    galn_hip_server_worker_mark();
     std::vector<Piece> pieces;
     std::vector<uint8_t> query;
     std::vector<uint64_t> qoff, off, pops, trace, bytes;
@@ -167,7 +177,10 @@ void worker_main(galn_hip_server *S)
         R.bopts = bopts.data();
         R.n_stacks = n_stacks.data();
         const auto t0 = Clock::now();
-        int r = S->rs ? galn_hip_readset_reload(S->rs, &R) : galn_hip_readset_upload(S->idx, &R, &S->rs);
+        int r = 0;
+        if (++S->n_launch == S->fail_at)
+            r = srv_fail(GALN_HIP_E_DEVICE, "launch %ld: failure injected by GALN_HIP_TEST_SERVER_FAIL", S->n_launch);
+        if (!r) r = S->rs ? galn_hip_readset_reload(S->rs, &R) : galn_hip_readset_upload(S->idx, &R, &S->rs);
         if (!r) r = galn_hip_readset_search(S->idx, S->rs, S->cfg.form, S->cfg.flags, nullptr);
         n_aln.resize(m), off.resize(m), pops.resize(m), trace.resize(m), bytes.resize(m);
         galn_hip_out O{};
@@ -212,6 +225,13 @@ void worker_main(galn_hip_server *S)
                     q.status = r;
                     q.err = err;
                 }
+                if (q.status < 0 && q.next < q.n) {
+                    // a failed request gets no further launches: out of the queue before its
+                    // status is published (result() may release it as soon as the lock drops)
+                    S->waiting.erase(std::remove(S->waiting.begin(), S->waiting.end(), &q), S->waiting.end());
+                    q.next = q.n;
+                }
+                if (q.status < 0) q.finished = q.next;
                 if (q.finished == q.n && q.status == 0) q.status = 1;
                 if (q.status != 0 && q.t_done == Clock::time_point{}) {
                     q.t_done = t1;
@@ -242,7 +262,9 @@ int galn_hip_server_open(galn_hip_index *idx, const galn_hip_server_cfg_t *cfg, 
     S->cfg.quantum_reads = std::min(S->cfg.quantum_reads, S->cfg.max_launch_reads);
     if (S->cfg.max_queued_bytes == 0) S->cfg.max_queued_bytes = 4ull << 30;
     if (S->cfg.form != 0) S->cfg.form = 1;
+    if (const char *v = getenv("GALN_HIP_TEST_SERVER_FAIL")) S->fail_at = strtol(v, nullptr, 10);
     galn_hip_index_hold(idx, 1);
+    galn_hip_index_serve(idx, 1);
     S->worker = std::thread(worker_main, S);
     *out = S;
     return 0;
@@ -328,7 +350,8 @@ int galn_hip_server_result(galn_hip_server *S, uint64_t id, galn_hip_out *out, d
     auto it = S->reqs.find(id);
     if (it == S->reqs.end()) return srv_fail(GALN_HIP_E_ARG, "server_result: unknown request %llu", (unsigned long long)id);
     Request *q = it->second.get();
-    if (q->status == 0) return srv_fail(GALN_HIP_E_BUSY, "server_result: request %llu not finished", (unsigned long long)id);
+    if (q->status == 0 || q->finished < q->next)
+        return srv_fail(GALN_HIP_E_BUSY, "server_result: request %llu not finished", (unsigned long long)id);
     int r = 0;
     if (q->status < 0) {
         r = q->status;
@@ -370,6 +393,7 @@ int galn_hip_server_close(galn_hip_server *S)
     S->cv_work.notify_all();
     if (S->worker.joinable()) S->worker.join();
     if (S->rs) galn_hip_readset_free(S->rs);
+    galn_hip_index_serve(S->idx, -1);
     galn_hip_index_hold(S->idx, -1);
     delete S;
     return 0;

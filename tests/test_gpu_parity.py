@@ -16,7 +16,7 @@ import pytest
 
 import fixtures as fx
 import oracle as orc
-from conftest import golden_inputs, golden_runs, load_reads, run_opts
+from conftest import golden_inputs, gpu_golden_runs, load_reads, run_opts
 from galn_amd import index as ix
 from galn_amd import native as nv
 
@@ -58,7 +58,7 @@ def readset_for(name, key, golden_dir):
 
 
 @pytest.mark.parametrize("form", list(FORMS))
-@pytest.mark.parametrize("name,key", golden_runs())
+@pytest.mark.parametrize("name,key", gpu_golden_runs())
 def test_golden_sai_pops_trace(name, key, form, golden_dir):
     print(f"case {name}/{key} form={form}", flush=True)
     w, o, rs = readset_for(name, key, golden_dir)
@@ -98,8 +98,12 @@ def test_push_cap_cuts_rounds_without_changing_results(name, key, must_cut, gold
 @pytest.mark.parametrize("form", ["plain", "batched"])
 @pytest.mark.parametrize("name,key", [("plain_small", "default_1"), ("adna_mixed", "adna_1"), ("deep_repeats", "deep_2"),
                                       ("options", "max_entries_m300_1"), ("fastq_edge", "adna_1"),
-                                      ("serving_mix", "default_1")])
+                                      ("serving_mix", "default_1"), ("beyond_limits", "default_1"),
+                                      ("beyond_limits", "default_2"), ("beyond_limits", "e50_1"),
+                                      ("beyond_limits", "n130_1")])
 def test_cli_writes_golden_sai(name, key, form, golden_dir, tmp_path):
+    """`galn aln` on the GPU writes the reference's .sai; reads outside the kernels' limits (long
+    reads, more than 256 score buckets, max_diff above 126) are finished by the CPU engine."""
     print(f"case cli {name}/{key} form={form}", flush=True)
     w = golden_inputs(name, golden_dir)
     opts, mate = run_opts(w["meta"], key)
@@ -236,3 +240,54 @@ def test_larger_reference_forms_agree_and_sample_matches_oracle(tmp_path):
     np.testing.assert_array_equal(rb.n_aln[:n], ref.n_aln)
     for i in range(n):
         np.testing.assert_array_equal(rb.records(i), ref.records(i))
+
+
+@pytest.mark.parametrize("form", ["plain", "batched"])
+def test_exhausted_pool_reruns_reads_with_fewer_lanes(form, golden_dir, monkeypatch):
+    """With the stack pool capped at 2048 granules (GALN_HIP_TEST_POOL_GRANULES) the first launch
+    runs out of granules: the reads that found the pool empty are re-run with a quarter of the
+    lanes (and of the wave engine's waves) until none is left, and records, pops and traces are
+    still the reference's."""
+    print(f"case pool cap adna_mixed/adna_1 form={form}", flush=True)
+    monkeypatch.setenv("GALN_HIP_TEST_POOL_GRANULES", "2048")
+    w = golden_inputs("adna_mixed", golden_dir)
+    opts, mate = run_opts(w["meta"], "adna_1")
+    r = load_reads(w["reads"][mate])
+    o = nv.gap_opt_from_args(opts)
+    bopts, nst, md = nv.prepare_batches(o, r["qlen"], r["batch"])
+    meta, words = ix.read_bwt(w["bwt"])
+    with nv.Index(words, meta, 0) as idx:  # a fresh index: its pool starts at the cap
+        rs = nv.ReadSet(idx, r["query"], r["qoff"], r["qlen"], md, r["batch"], bopts, nst)
+        rs.search(FORMS[form][0], nv.GALN_HIP_TRACE)
+        res = rs.results(want_pops=True, want_trace=True)
+        tiers = [l["tier"] for l in rs.launches()]
+        rs.close()
+    exp = w["expected"]
+    np.testing.assert_array_equal(res.n_aln, exp["adna_1_n_aln"])
+    np.testing.assert_array_equal(res.pops, exp["adna_1_pops"])
+    np.testing.assert_array_equal(res.trace, exp["adna_1_trace"])
+    assert fx.md5_bytes(nv.sai_bytes(o, res.n_aln, res.rec_off, res.rec)) == w["meta"]["runs"]["adna_1"]["sai_md5"]
+    assert any(t not in (0, 10) for t in tiers), f"no re-run launch: {tiers}"
+
+
+def test_index_close_is_refused_while_a_read_set_is_open(golden_dir):
+    """galn_hip_index_close returns GALN_HIP_E_BUSY and keeps the index while one of its read
+    sets is open (freeing the read set afterwards used its index's stream), and succeeds once
+    the read set is freed."""
+    print("case index close order", flush=True)
+    w = golden_inputs("plain_small", golden_dir)
+    opts, mate = run_opts(w["meta"], "default_1")
+    r = load_reads(w["reads"][mate])
+    o = nv.gap_opt_from_args(opts)
+    bopts, nst, md = nv.prepare_batches(o, r["qlen"], r["batch"])
+    meta, words = ix.read_bwt(w["bwt"])
+    idx = nv.Index(words, meta, 0)
+    rs = nv.ReadSet(idx, r["query"], r["qoff"], r["qlen"], md, r["batch"], bopts, nst)
+    L = nv.hip_lib()
+    assert L.galn_hip_index_close(idx.h) == -6
+    rs.search(nv.FORM_BATCHED)  # the index is still usable
+    res = rs.results()
+    assert fx.md5_bytes(nv.sai_bytes(o, res.n_aln, res.rec_off, res.rec)) == w["meta"]["runs"]["default_1"]["sai_md5"]
+    rs.close()
+    assert L.galn_hip_index_close(idx.h) == 0
+    idx.h = None

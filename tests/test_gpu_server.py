@@ -95,3 +95,53 @@ def test_admission_limit_and_release(golden_dir):
             assert b is not None and srv.wait(b, 60000)
             rb, _ = srv.result(b)
     assert ra.n_aln.size == 1490 and rb.n_aln.size == 1
+
+
+def test_failed_launch_releases_request_and_server_goes_on(golden_dir, monkeypatch):
+    """A launch that fails (injected on the host, before the device sees it) fails every request
+    in it; a request spread over several launches is taken out of the queue before its error is
+    published, so collecting it while its later reads were still queued is safe, and the server
+    keeps serving the requests after it."""
+    print("case server failed launch", flush=True)
+    w, reqs = requests(golden_dir)
+    meta, words = ix.read_bwt(w["bwt"])
+    big = reqs[7][3]  # 1490 reads: six launches of 256
+    key, lo, hi, small = reqs[4]
+    monkeypatch.setenv("GALN_HIP_TEST_SERVER_FAIL", "2")
+    with nv.Index(words, meta, 0) as idx:
+        with nv.Server(idx, max_launch_reads=256, quantum_reads=64, flags=nv.GALN_HIP_TRACE) as srv:
+            a = srv.submit(*big)
+            with pytest.raises(nv.GalnError, match="injected"):
+                srv.wait(a, 60000)
+            with pytest.raises(nv.GalnError, match="injected"):
+                srv.result(a)
+            b = srv.submit(*small)
+            assert srv.wait(b, 60000)
+            rb, _ = srv.result(b, want_trace=True)
+            st = srv.stats()
+    exp = w["expected"]
+    np.testing.assert_array_equal(rb.n_aln, exp[key + "_n_aln"][lo:hi])
+    np.testing.assert_array_equal(rb.pops, exp[key + "_pops"][lo:hi])
+    np.testing.assert_array_equal(rb.trace, exp[key + "_trace"][lo:hi])
+    # the failed request's later reads were never launched
+    assert st["reads"] < len(big[2]) + (hi - lo)
+
+
+def test_index_is_busy_while_a_server_holds_it(golden_dir):
+    """While a server is open, direct searches on its index are refused (GALN_HIP_E_BUSY): the
+    index's stream and stack pool belong to the server's worker. After close they work again."""
+    print("case server busy index", flush=True)
+    w, reqs = requests(golden_dir)
+    meta, words = ix.read_bwt(w["bwt"])
+    arrays = reqs[2][3]
+    with nv.Index(words, meta, 0) as idx:
+        srv = nv.Server(idx)
+        with pytest.raises(nv.GalnError, match=r"failed \(-6\)"):
+            nv.ReadSet(idx, *arrays)
+        rid = srv.submit(*arrays)
+        assert srv.wait(rid, 60000)
+        srv.result(rid)
+        srv.close()
+        rs = nv.ReadSet(idx, *arrays)
+        rs.search(nv.FORM_BATCHED)
+        rs.close()
