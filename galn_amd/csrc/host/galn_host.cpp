@@ -1,6 +1,7 @@
 // galn_host.cpp -- host side of `galn aln` (see include/galn_host.h): option parsing,
 // per-batch options, .bwt loading, FASTQ/FASTA input, .sai output, and the aln driver
-// that hands the reads to the GPU search (galn_hip.h). No search runs on the CPU.
+// that hands the reads to the GPU search (galn_hip.h), and to the CPU engine (galn_cpu.h) for
+// --cpu-only and for the reads outside the GPU kernels' limits.
 
 #include <cctype>
 #include <chrono>
@@ -19,6 +20,7 @@
 #include <unistd.h>
 #include <zlib.h>
 
+#include "galn_cpu.h"
 #include "galn_hip.h"
 #include "galn_host.h"
 
@@ -49,8 +51,8 @@ extern "C" void galn_gap_opt_default(galn_gap_opt_t *o)
 
 // options of the reference that change nothing here (CPU/GPU sharing and tuning of its own
 // engines); accepted so existing command lines keep working
-static const char *k_ignored_valued[] = {"--cpu-workers", "--cpu-interleave", "--handoff", "--handoff-when",
-                                         "--gpus", "--light-cap", "--warps", "--pool-gb", "--alloc", nullptr};
+static const char *k_ignored_valued[] = {"--handoff", "--handoff-when", "--gpus", "--light-cap", "--warps",
+                                         "--pool-gb", "--alloc", nullptr};
 
 extern "C" int galn_parse_aln_args(int argc, char **argv, galn_aln_args_t *a)
 {
@@ -78,14 +80,8 @@ extern "C" int galn_parse_aln_args(int argc, char **argv, galn_aln_args_t *a)
         if (!strcmp(s, "--gpu-only")) continue;  // the GPU always does every read here
         if (!strcmp(s, "-v")) { a->verbose = 1; continue; }
         if (!strcmp(s, "--trace")) { a->trace = 1; continue; }
-        if (!strcmp(s, "--cpu-only")) {
-            fprintf(stderr, "[galn] --cpu-only: this build searches on the GPU only\n");
-            return -1;
-        }
-        if (!strcmp(s, "--verify")) {
-            fprintf(stderr, "[galn] --verify: this build has no CPU search engine to verify against\n");
-            return -1;
-        }
+        if (!strcmp(s, "--cpu-only")) { a->cpu_only = 1; continue; }
+        if (!strcmp(s, "--verify")) { a->verify = 1; continue; }
         if (!v) {
             fprintf(stderr, "[galn] option %s needs a value\n", s);
             return -1;
@@ -117,6 +113,8 @@ extern "C" int galn_parse_aln_args(int argc, char **argv, galn_aln_args_t *a)
             }
         }
         else if (!strcmp(s, "--device")) a->device = atoi(v);
+        else if (!strcmp(s, "--cpu-workers")) a->cpu_threads = atoi(v);
+        else if (!strcmp(s, "--cpu-interleave")) a->cpu_interleave = atoi(v);
         else if (!strcmp(s, "--form")) {
             if (!strcmp(v, "plain")) a->form = GALN_FORM_PLAIN;
             else if (!strcmp(v, "batched")) a->form = GALN_FORM_BATCHED;
@@ -524,12 +522,64 @@ static int usage()
             "         -N        non-iterative mode: search for all n-difference hits (slooow)\n"
             "         -f FILE   file to write output to instead of stdout\n"
             "         (not supported: -q, -b/-0/-1/-2, -I, -Y, -B)\n\n"
-            "GPU options:\n"
+            "GPU / CPU options:\n"
             "         --device INT       HIP device to use [0]\n"
             "         --form STR         plain (one launch per 0x40000 reads) or batched (persistent\n"
             "                            kernel with a device work queue) [batched]\n"
+            "         --cpu-only         search every read on the CPU (no GPU)\n"
+            "         --gpu-only         the GPU searches every read it can; the CPU only those outside\n"
+            "                            the GPU kernels' limits (the default here)\n"
+            "         --cpu-workers INT  CPU search threads [all hardware threads]\n"
+            "         --cpu-interleave INT  reads one CPU thread advances in turn [8]\n"
+            "         --verify           re-run every read on the CPU and compare (exit 6 on a difference)\n"
             "         -v                 print timing to stderr\n\n");
     return 1;
+}
+
+// Records of a search: per read n_aln, offset, pops, trace; the record array.
+struct SearchOut {
+    std::vector<int32_t> n_aln;
+    std::vector<uint64_t> rec_off, pops, trace;
+    std::vector<galn_aln_rec_t> rec;
+    void resize(int n)
+    {
+        const size_t m = (size_t)std::max(n, 1);
+        n_aln.assign(m, 0);
+        rec_off.assign(m, 0);
+        pops.assign(m, 0);
+        trace.assign(m, 0);
+        if (rec.size() < 2 * m) rec.resize(2 * m);
+    }
+    galn_hip_out view(bool want_trace)
+    {
+        galn_hip_out O;
+        O.n_aln = n_aln.data();
+        O.rec_off = rec_off.data();
+        O.rec = rec.data();
+        O.rec_cap = rec.size();
+        O.n_rec = 0;
+        O.n_pops = pops.data();
+        O.trace = want_trace ? trace.data() : nullptr;
+        O.occ_bytes = nullptr;
+        return O;
+    }
+};
+
+// The CPU engine over `list` (or every read), growing the record array when it is too small.
+static int cpu_run(const uint32_t *bwt, const galn_bwt_meta_t *meta, const galn_hip_reads *R, const std::vector<int32_t> *list,
+                   const galn_aln_args_t &A, SearchOut &so)
+{
+    const int n = list ? (int)list->size() : R->n_reads;
+    so.resize(n);
+    for (;;) {
+        galn_hip_out O = so.view(true);
+        const int r = galn_cpu_search(bwt, meta, R, list ? list->data() : nullptr, n, &O, A.cpu_threads, A.cpu_interleave);
+        if (r == GALN_HIP_E_RECCAP) {
+            so.rec.resize(O.n_rec);
+            continue;
+        }
+        return r;
+    }
 }
 
 extern "C" int galn_aln_main(int argc, char **argv)
@@ -563,15 +613,6 @@ extern "C" int galn_aln_main(int argc, char **argv)
     std::vector<int32_t> nstacks(nb), maxdiff(std::max(n, 1));
     galn_prepare_batches(&A.opt, n, galn_readbuf_qlen(reads), galn_readbuf_batch(reads), nb, bopts.data(), nstacks.data(),
                          maxdiff.data());
-
-    galn_hip_index *idx = nullptr;
-    int r = galn_hip_index_open(bwt, &meta, A.device, &idx);
-    if (r) {
-        fprintf(stderr, "[galn] GPU: %s\n", galn_hip_last_error());
-        galn_free_bwt(bwt);
-        galn_readbuf_free(reads);
-        return 10;
-    }
     galn_hip_reads R;
     R.n_reads = n;
     R.query = galn_readbuf_query(reads);
@@ -582,57 +623,133 @@ extern "C" int galn_aln_main(int argc, char **argv)
     R.n_batches = nb;
     R.bopts = bopts.data();
     R.n_stacks = nstacks.data();
-    std::vector<int32_t> n_aln(std::max(n, 1));
-    std::vector<uint64_t> rec_off(std::max(n, 1)), pops(std::max(n, 1)), trace(std::max(n, 1));
-    std::vector<galn_aln_rec_t> rec((size_t)std::max(n, 1) * 2);
-    galn_hip_out O;
-    O.n_aln = n_aln.data();
-    O.rec_off = rec_off.data();
-    O.rec = rec.data();
-    O.rec_cap = rec.size();
-    O.n_rec = 0;
-    O.n_pops = pops.data();
-    O.trace = A.trace ? trace.data() : nullptr;
-    O.occ_bytes = nullptr;
-    const uint32_t flags = A.trace ? GALN_HIP_TRACE : 0u;
-    galn_hip_readset *rs = nullptr;
-    r = galn_hip_readset_upload(idx, &R, &rs);
-    if (!r) r = galn_hip_readset_search(idx, rs, A.form, flags, nullptr);
-    if (!r) {
-        r = galn_hip_readset_results(rs, &O);
-        if (r == GALN_HIP_E_RECCAP) {
-            rec.resize(O.n_rec);
-            O.rec = rec.data();
-            O.rec_cap = rec.size();
-            r = galn_hip_readset_results(rs, &O);
-        }
-    }
-    double kfirst = 0, ktotal = 0;
-    if (rs) galn_hip_readset_kernel_ms(rs, &kfirst, &ktotal);
-    if (r) {
-        fprintf(stderr, "[galn] GPU search failed: %s\n", galn_hip_last_error());
-        galn_hip_readset_free(rs);
-        galn_hip_index_close(idx);
+    auto cleanup = [&]() {
         galn_free_bwt(bwt);
         galn_readbuf_free(reads);
-        return 10;
+    };
+
+    SearchOut res;
+    double kfirst = 0, ktotal = 0, t_cpu = 0;
+    int n_cpu = 0;
+    int r = 0;
+    if (A.cpu_only) {
+        // every read on the CPU engine (src/galn.cu:450-455: no device is opened)
+        const double tc = now_s();
+        r = cpu_run(bwt, &meta, &R, nullptr, A, res);
+        t_cpu = now_s() - tc;
+        n_cpu = n;
+        if (r) {
+            fprintf(stderr, "[galn] CPU search failed (%d)\n", r);
+            cleanup();
+            return 10;
+        }
+    } else {
+        galn_hip_index *idx = nullptr;
+        r = galn_hip_index_open(bwt, &meta, A.device, &idx);
+        if (r) {
+            fprintf(stderr, "[galn] GPU: %s (use --cpu-only to run without a GPU)\n", galn_hip_last_error());
+            cleanup();
+            return 10;
+        }
+        res.resize(n);
+        const bool want_trace = A.trace || A.verify;
+        galn_hip_out O = res.view(want_trace);
+        const uint32_t flags = want_trace ? GALN_HIP_TRACE : 0u;
+        galn_hip_readset *rs = nullptr;
+        r = galn_hip_readset_upload(idx, &R, &rs);
+        if (!r) r = galn_hip_readset_search(idx, rs, A.form, flags, nullptr);
+        if (!r) {
+            r = galn_hip_readset_results(rs, &O);
+            if (r == GALN_HIP_E_RECCAP) {
+                res.rec.resize(O.n_rec);
+                O = res.view(want_trace);
+                r = galn_hip_readset_results(rs, &O);
+            }
+        }
+        if (rs) galn_hip_readset_kernel_ms(rs, &kfirst, &ktotal);
+        if (r) {
+            fprintf(stderr, "[galn] GPU search failed: %s\n", galn_hip_last_error());
+            galn_hip_readset_free(rs);
+            galn_hip_index_close(idx);
+            cleanup();
+            return 10;
+        }
+        galn_hip_readset_free(rs);
+        galn_hip_index_close(idx);
+        res.rec.resize(O.n_rec);
+        // the reads outside the kernels' limits finish on the CPU engine, as the reference hands
+        // them to its CPU workers (src/gapw.cuh:753-755, src/galn.cu:1151-1180)
+        std::vector<int32_t> cpu_list;
+        for (int i = 0; i < n; ++i)
+            if (res.n_aln[(size_t)i] == GALN_READ_ERR_UNSUP) cpu_list.push_back(i);
+        if (!cpu_list.empty()) {
+            const double tc = now_s();
+            SearchOut c;
+            r = cpu_run(bwt, &meta, &R, &cpu_list, A, c);
+            if (r) {
+                fprintf(stderr, "[galn] CPU search failed (%d)\n", r);
+                cleanup();
+                return 10;
+            }
+            // merge: records of every read in read order, from whichever engine searched it
+            std::vector<galn_aln_rec_t> rec;
+            rec.reserve(res.rec.size() + c.rec.size());
+            size_t j = 0;
+            for (int i = 0; i < n; ++i) {
+                const bool on_cpu = j < cpu_list.size() && cpu_list[j] == i;
+                const galn_aln_rec_t *src = on_cpu ? c.rec.data() + c.rec_off[j] : res.rec.data() + res.rec_off[(size_t)i];
+                if (on_cpu) {
+                    res.n_aln[(size_t)i] = c.n_aln[j];
+                    res.pops[(size_t)i] = c.pops[j];
+                    res.trace[(size_t)i] = c.trace[j];
+                    ++j;
+                }
+                res.rec_off[(size_t)i] = rec.size();
+                if (res.n_aln[(size_t)i] > 0) rec.insert(rec.end(), src, src + res.n_aln[(size_t)i]);
+            }
+            res.rec.swap(rec);
+            n_cpu = (int)cpu_list.size();
+            t_cpu = now_s() - tc;
+        }
     }
-    galn_hip_readset_free(rs);
-    galn_hip_index_close(idx);
     const double t_search = now_s();
+
+    int verify_fail = 0;
+    if (A.verify && !A.cpu_only) {
+        // every GPU read again on the CPU engine: records, pops and traces must agree
+        // (src/galn.cu:1222-1248 re-runs the reads of its warp engine the same way)
+        SearchOut c;
+        r = cpu_run(bwt, &meta, &R, nullptr, A, c);
+        long bad = 0;
+        for (int i = 0; i < n && !r; ++i) {
+            const int na = res.n_aln[(size_t)i];
+            bool ok = c.n_aln[(size_t)i] == na && c.pops[(size_t)i] == res.pops[(size_t)i] &&
+                      c.trace[(size_t)i] == res.trace[(size_t)i];
+            if (ok && na > 0)
+                ok = !memcmp(c.rec.data() + c.rec_off[(size_t)i], res.rec.data() + res.rec_off[(size_t)i],
+                             (size_t)na * sizeof(galn_aln_rec_t));
+            if (!ok && bad++ < 10)
+                fprintf(stderr, "[galn verify] read %d (%s) differs: cpu n=%d pops=%llu | gpu n=%d pops=%llu\n", i,
+                        galn_readbuf_name(reads, i), c.n_aln[(size_t)i], (unsigned long long)c.pops[(size_t)i], na,
+                        (unsigned long long)res.pops[(size_t)i]);
+        }
+        if (r) bad = 1;
+        fprintf(stderr, "[galn verify] %d reads re-run on the CPU: %ld differ -> %s\n", n, bad, bad ? "FAIL" : "PASS");
+        verify_fail = bad ? 6 : 0;
+    }
 
     int err = 0;
     long mapped = 0, n_rec = 0;
     for (int i = 0; i < n; ++i) {
-        if (n_aln[i] < 0) {
-            fprintf(stderr, "[galn] read %d (%s): search error %d\n", i, galn_readbuf_name(reads, i), n_aln[i]);
+        if (res.n_aln[(size_t)i] < 0) {
+            fprintf(stderr, "[galn] read %d (%s): search error %d\n", i, galn_readbuf_name(reads, i), res.n_aln[(size_t)i]);
             err = 4;
-        } else if (n_aln[i] > 0) {
+        } else if (res.n_aln[(size_t)i] > 0) {
             ++mapped;
-            n_rec += n_aln[i];
+            n_rec += res.n_aln[(size_t)i];
         }
     }
-    if (galn_sai_write(out, &A.opt, n, n_aln.data(), rec_off.data(), rec.data())) {
+    if (galn_sai_write(out, &A.opt, n, res.n_aln.data(), res.rec_off.data(), res.rec.data())) {
         fprintf(stderr, "[galn] error writing the .sai\n");
         err = 5;
     }
@@ -642,16 +759,19 @@ extern "C" int galn_aln_main(int argc, char **argv)
             mapped, n_rec, t_end - t0, t_loaded - t0, t_search - t_loaded, t_end - t_search);
     if (A.verbose) {
         double pops_total = 0;
-        for (int i = 0; i < n; ++i) pops_total += (double)pops[i];
-        fprintf(stderr, "[galn] GPU %d, %s form: kernels %.3f s (first tier %.3f s), %.3e pops, %.3e reads/s\n", A.device,
-                A.form == GALN_FORM_PLAIN ? "plain" : "batched", ktotal / 1e3, kfirst / 1e3, pops_total,
-                ktotal > 0 ? n / (ktotal / 1e3) : 0.0);
+        for (int i = 0; i < n; ++i) pops_total += (double)res.pops[(size_t)i];
+        if (!A.cpu_only)
+            fprintf(stderr, "[galn] GPU %d, %s form: kernels %.3f s (first tier %.3f s), %.3e pops, %.3e reads/s\n",
+                    A.device, A.form == GALN_FORM_PLAIN ? "plain" : "batched", ktotal / 1e3, kfirst / 1e3, pops_total,
+                    ktotal > 0 ? n / (ktotal / 1e3) : 0.0);
+        if (n_cpu)
+            fprintf(stderr, "[galn] CPU engine: %d reads in %.3f s (%.3e reads/s)\n", n_cpu, t_cpu,
+                    t_cpu > 0 ? n_cpu / t_cpu : 0.0);
         if (A.trace)
             for (int i = 0; i < n && i < 5; ++i)
-                fprintf(stderr, "[galn] read %d: n_aln %d pops %llu trace %016llx\n", i, n_aln[i],
-                        (unsigned long long)pops[i], (unsigned long long)trace[i]);
+                fprintf(stderr, "[galn] read %d: n_aln %d pops %llu trace %016llx\n", i, res.n_aln[(size_t)i],
+                        (unsigned long long)res.pops[(size_t)i], (unsigned long long)res.trace[(size_t)i]);
     }
-    galn_free_bwt(bwt);
-    galn_readbuf_free(reads);
-    return err;
+    cleanup();
+    return err ? err : verify_fail;
 }

@@ -36,7 +36,7 @@ HOST_SYMBOLS = (
     "galn_gap_opt_default", "galn_parse_aln_args", "galn_cal_maxdiff", "galn_batch_opt", "galn_read_maxdiff",
     "galn_prepare_batches", "galn_load_bwt", "galn_free_bwt", "galn_read_fastq", "galn_readbuf_n",
     "galn_readbuf_n_batches", "galn_readbuf_query", "galn_readbuf_qoff", "galn_readbuf_qlen", "galn_readbuf_batch",
-    "galn_readbuf_name", "galn_readbuf_free", "galn_sai_write", "galn_aln_main",
+    "galn_readbuf_name", "galn_readbuf_free", "galn_sai_write", "galn_aln_main", "galn_cpu_search",
 )
 
 
@@ -90,7 +90,8 @@ class GapOpt(C.Structure):
 
 class AlnArgs(C.Structure):
     _fields_ = [("opt", GapOpt), ("prefix", C.c_char_p), ("reads", C.c_char_p), ("out", C.c_char_p),
-                ("device", C.c_int), ("form", C.c_int), ("verbose", C.c_int), ("trace", C.c_int)]
+                ("device", C.c_int), ("form", C.c_int), ("verbose", C.c_int), ("trace", C.c_int),
+                ("cpu_only", C.c_int), ("verify", C.c_int), ("cpu_threads", C.c_int), ("cpu_interleave", C.c_int)]
 
 
 class GalnError(RuntimeError):
@@ -164,6 +165,8 @@ def host_lib():
         L.galn_readbuf_name.argtypes = [P, C.c_int32]
         L.galn_readbuf_free.argtypes = [P]
         L.galn_aln_main.argtypes = [C.c_int, C.POINTER(C.c_char_p)]
+        L.galn_cpu_search.argtypes = [P, C.POINTER(BwtMeta), C.POINTER(Reads), P, C.c_int32, C.POINTER(Out),
+                                      C.c_int32, C.c_int32]
         _host = L
     return _host
 
@@ -562,6 +565,36 @@ class Server:
             self.close()
         except Exception:
             pass
+
+
+def cpu_search(words: np.ndarray, meta: BwtMeta | dict, query, qoff, qlen, max_diff, batch, bopts, n_stacks,
+               reads_list: np.ndarray | None = None, threads: int = 0, interleave: int = 8) -> Results:
+    """galn_cpu_search: the CPU engine over all reads, or over the read ids in `reads_list`
+    (results then by list position)."""
+    m = meta if isinstance(meta, BwtMeta) else meta_from_dict(meta)
+    w = np.ascontiguousarray(words, dtype=np.uint32)
+    R, keep = make_reads(query, qoff, qlen, max_diff, batch, bopts, n_stacks)
+    lst = None if reads_list is None else np.ascontiguousarray(reads_list, dtype=np.int32)
+    n = int(R.n_reads) if lst is None else int(lst.size)
+    n_aln = np.zeros(max(n, 1), np.int32)
+    rec_off = np.zeros(max(n, 1), np.uint64)
+    pops = np.zeros(max(n, 1), np.uint64)
+    trace = np.zeros(max(n, 1), np.uint64)
+    ob = np.zeros(max(n, 1), np.uint64)
+    cap = 2 * n + 16
+    while True:
+        rec = np.zeros((max(cap, 1), 3), np.uint64)
+        O = Out()
+        O.n_aln, O.rec_off, O.rec, O.rec_cap = _p(n_aln), _p(rec_off), _p(rec), cap
+        O.n_pops, O.trace, O.occ_bytes = _p(pops), _p(trace), _p(ob)
+        rc = host_lib().galn_cpu_search(_p(w), C.byref(m), C.byref(R), _p(lst) if lst is not None else None,
+                                        n, C.byref(O), int(threads), int(interleave))
+        if rc == E_RECCAP:
+            cap = int(O.n_rec)
+            continue
+        if rc != 0:
+            raise GalnError(f"cpu_search failed ({rc})")
+        return Results(n_aln[:n], rec_off[:n], rec[:int(O.n_rec)], pops[:n], trace[:n], ob[:n])
 
 
 def sai_bytes(opt: GapOpt, n_aln: np.ndarray, rec_off: np.ndarray, rec: np.ndarray) -> bytes:

@@ -1,7 +1,8 @@
 /* galn_host.h -- host side of `galn aln`: options, index and read input, .sai output.
  *
- * Everything here runs on the CPU and does no search; the search runs on the GPU
- * through galn_hip.h. Interfaces this replaces in the reference (bilalbioinfo/galn 0.2.7):
+ * Everything here runs on the CPU; the search runs on the GPU through galn_hip.h, and on the
+ * CPU engine (galn_cpu.h) for --cpu-only and for the reads outside the GPU kernels' limits.
+ * Interfaces this replaces in the reference (bilalbioinfo/galn 0.2.7):
  *   galn_gap_opt_t / galn_gap_opt_default   sai_gap_opt_t, opt_default   src/sai_format.h:17-27, src/opt_build.h:23-36
  *   galn_parse_aln_args                     parse_args                   src/galn.cu:185-250
  *   galn_cal_maxdiff                        aln_cal_maxdiff              src/aln_host.c:9-21
@@ -55,6 +56,11 @@ typedef struct {
     int form;               /* --form plain|batched */
     int verbose;            /* -v */
     int trace;              /* --trace: also compute per-read trace hashes (printed with -v) */
+    int cpu_only;           /* --cpu-only: every read on the CPU engine, no GPU */
+    int verify;             /* --verify: re-run the GPU's reads on the CPU engine and compare
+                               records, pops and traces (exit 6 on a difference) */
+    int cpu_threads;        /* --cpu-workers N: CPU engine threads [all hardware threads] */
+    int cpu_interleave;     /* --cpu-interleave N: reads a CPU thread advances in turn [8] */
 } galn_aln_args_t;
 
 void galn_gap_opt_default(galn_gap_opt_t *o);

@@ -55,6 +55,16 @@ def golden_runs():
     return out
 
 
+# fixtures whose reads or options lie outside the GPU kernels' limits: the kernels report those
+# reads as unsupported and `galn aln` finishes them on the CPU engine
+BEYOND_GPU_LIMITS = ("beyond_limits",)
+
+
+def gpu_golden_runs():
+    """(fixture, run key) for every golden run the GPU kernels search whole."""
+    return [(n, k) for n, k in golden_runs() if n not in BEYOND_GPU_LIMITS]
+
+
 def run_opts(meta: dict, key: str) -> tuple[list[str], int]:
     r = meta["runs"][key]
     return list(r["opts"]), int(r["reads"])

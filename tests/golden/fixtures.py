@@ -68,7 +68,19 @@ SPECS = {
         "ref": ("plain", 71, 200_000), "reads": ("file", "edge.fq"),
         "opts": {"adna": ADNA, "default": []},
     },
+    # beyond the GPU kernels' limits: more than 256 score buckets (-e 50), max_diff above 126
+    # (-n 130), reads of 1024-2000 bp (the second read file); an option list given with read
+    # file numbers runs on those files only
+    "beyond_limits": {
+        "ref": ("plain", 81, 500_000), "reads": ("two_uniform", 82, 300, 30, 150, 24, 1024, 2000),
+        "opts": {"default": [], "e50": (["-o", "2", "-e", "50"], [0]), "n130": (["-n", "130"], [0])},
+    },
 }
+
+
+def opts_and_files(v) -> tuple[list[str], list[int] | None]:
+    """An option list of a spec, and the read files it runs on (None: all)."""
+    return (list(v), None) if isinstance(v, list) else (list(v[0]), list(v[1]))
 
 
 def md5_file(path: str) -> str:
@@ -118,6 +130,10 @@ def build(name: str, workdir: str, device: str = "cpu") -> dict:
     elif kind == "uniform":
         r, L = synth.uniform_length_reads(spec["reads"][1], ref, spec["reads"][2], spec["reads"][3], spec["reads"][4])
         fqs.append((r, L))
+    elif kind == "two_uniform":
+        sd, n1, lo1, hi1, n2, lo2, hi2 = spec["reads"][1:]
+        fqs.append(synth.uniform_length_reads(sd, ref, n1, lo1, hi1, sub_rate=0.01, deam=0.0))
+        fqs.append(synth.uniform_length_reads(sd + 1, ref, n2, lo2, hi2, sub_rate=0.01, deam=0.0))
     elif kind == "paired":
         m1, l1, m2, l2 = synth.paired_reads(spec["reads"][1], ref, spec["reads"][2])
         fqs += [(m1, l1), (m2, l2)]

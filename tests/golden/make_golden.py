@@ -112,8 +112,11 @@ def make(name: str) -> None:
         meta = {"name": name, "spec": w["spec"], "bwt_md5": bwt_md5,
                 "reads_md5": [fx.md5_file(p) for p in w["reads"]], "runs": {}}
         arrs = {}
-        for oname, opts in w["spec"]["opts"].items():
+        for oname, v in w["spec"]["opts"].items():
+            opts, files = fx.opts_and_files(v)
             for mi, rp in enumerate(w["reads"]):
+                if files is not None and mi not in files:
+                    continue
                 key = f"{oname}_{mi + 1}"
                 sai = os.path.join(tmp, key + ".sai")
                 st = os.path.join(tmp, key + ".tsv")

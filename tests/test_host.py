@@ -26,15 +26,22 @@ def header_functions(path: str) -> list[str]:
     return sorted(set(re.findall(r"^\s*(?:const\s+)?[a-z_0-9]+\s*\*?\s*(galn_[a-z_0-9]+)\s*\(", txt, flags=re.M)))
 
 
-@pytest.mark.parametrize("header,lib", [("galn_hip.h", nv.HIP_LIB), ("galn_host.h", nv.HOST_LIB)])
+HEADERS = {"galn_hip.h": nv.HIP_LIB, "galn_host.h": nv.HOST_LIB, "galn_cpu.h": nv.HOST_LIB}
+
+
+@pytest.mark.parametrize("header,lib", list(HEADERS.items()))
 def test_libraries_load_and_export_every_declared_function(header, lib):
     names = header_functions(os.path.join(ROOT, "include", header))
-    assert len(names) >= 10
+    assert len(names) >= 1
     L = C.CDLL(lib)
     missing = [n for n in names if not hasattr(L, n)]
     assert not missing, missing
-    # and the ctypes mirror binds every one of them
-    mirrored = nv.HIP_SYMBOLS if header == "galn_hip.h" else nv.HOST_SYMBOLS
+
+
+@pytest.mark.parametrize("lib,mirrored", [(nv.HIP_LIB, nv.HIP_SYMBOLS), (nv.HOST_LIB, nv.HOST_SYMBOLS)])
+def test_ctypes_mirror_binds_every_declared_function(lib, mirrored):
+    names = sorted(n for h, l in HEADERS.items() if l == lib
+                   for n in header_functions(os.path.join(ROOT, "include", h)))
     assert sorted(mirrored) == names
 
 
@@ -71,10 +78,18 @@ def test_defaults_are_bwa_defaults():
 
 
 @pytest.mark.parametrize("args", [["p"], ["p", "r", "x"], ["-q", "5", "p", "r"], ["--bogus", "p", "r"], ["p", "r", "-n"],
-                                  ["--cpu-only", "p", "r"], ["--form", "fast", "p", "r"]])
+                                  ["--cpu-workers", "p", "r"], ["--form", "fast", "p", "r"]])
 def test_option_errors(args):
     rc, _ = nv.parse_aln_args(args)
     assert rc != 0
+
+
+def test_engine_options_parse():
+    rc, a = nv.parse_aln_args(["--cpu-only", "--verify", "--cpu-workers", "3", "--cpu-interleave", "5", "--gpu-only",
+                               "p", "r"])
+    assert rc == 0 and (a.cpu_only, a.verify, a.cpu_threads, a.cpu_interleave) == (1, 1, 3, 5)
+    rc, a = nv.parse_aln_args(["p", "r"])
+    assert rc == 0 and (a.cpu_only, a.verify, a.cpu_threads, a.cpu_interleave) == (0, 0, 0, 0)
 
 
 def test_reference_tuning_flags_accepted():
@@ -194,14 +209,15 @@ def test_cli_version_and_usage():
 
 def test_cli_exit_codes(tmp_path, golden_dir):
     w = golden_inputs("plain_small", golden_dir)
-    assert _galn("aln", "--cpu-only", w["prefix"], w["reads"][0]).returncode == 1
+    assert _galn("aln", "--cpu-workers", w["prefix"], w["reads"][0]).returncode == 1
     assert _galn("aln", str(tmp_path / "nope"), w["reads"][0]).returncode == 2
     assert _galn("aln", w["prefix"], str(tmp_path / "nope.fq")).returncode == 3
     assert _galn("aln", "-f", "/nonexistent/dir/x.sai", w["prefix"], w["reads"][0]).returncode == 1
 
 
 def test_cli_without_gpu_fails_loudly(golden_dir):
-    """No silent CPU fallback: with no HIP device the search is refused (exit 10)."""
+    """No silent CPU fallback: with no HIP device the GPU search is refused (exit 10); only
+    --cpu-only searches on the CPU."""
     import torch
     if torch.cuda.is_available():
         pytest.skip("a GPU is visible")
