@@ -3090,6 +3090,8 @@ __attribute__((visibility("hidden"))) void galn_hip_server_worker_mark() { tl_se
 __attribute__((visibility("hidden"))) void galn_hip_index_serve(galn_hip_index *idx, int delta) { idx->n_servers += delta; }
 
 namespace {
+int noalloc_setup(int device);
+
 int busy_check(const galn_hip_index *idx, const char *what)
 {
     if (idx && idx->n_servers.load() > 0 && !tl_server_worker)
@@ -3147,6 +3149,9 @@ int galn_hip_index_open(const uint32_t *bwt, const galn_bwt_meta_t *meta, int de
         delete idx;
         return fail(GALN_HIP_E_DEVICE, "index upload failed");
     }
+    // device address of the no-allocation search's array on this device (once per device); a
+    // failure here is reported by that entry point, which looks it up again
+    if (noalloc_setup(device) != 0) g_err[0] = 0;
     *out = idx;
     return 0;
 }
@@ -3393,3 +3398,268 @@ int galn_hip_search_batched(galn_hip_index *idx, const galn_hip_reads *reads, ga
 }
 
 }  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Search without allocation (galn_hip_search_device_noalloc). Every buffer the search needs
+// beyond the caller's inputs and outputs -- counters, failure lists, per-batch options, the
+// lanes' regions (widths, record slots, granule lists) and the stack pool -- is carved out of
+// one static device array, `scratch`, which each GPU holds from the library's first use on it
+// until the process ends. Host code never passes the array's name to a kernel: its device
+// address on each GPU comes from hipGetSymbolAddress, taken once per device (with that device
+// current) when the first index is opened there.
+
+namespace {
+constexpr size_t NA_SCRATCH_BYTES = 256ull << 20;  // the array's size
+constexpr int NA_MAX_READS = 1 << 20;              // reads per call
+constexpr int NA_MAX_BATCHES = 4096;               // per-batch option sets per call
+constexpr int NA_MAX_DEVICES = 64;
+}  // namespace
+
+static __device__ __align__(256) uint8_t scratch[NA_SCRATCH_BYTES];
+
+namespace {
+
+uint8_t *g_na_ptr[NA_MAX_DEVICES];  // device address of `scratch` on each device
+std::mutex g_na_setup_mu;
+std::mutex g_na_mu[NA_MAX_DEVICES];  // calls on one device take turns (they share its array)
+
+// Looks up the device address of `scratch` on `device`, which must be current. Idempotent.
+int noalloc_setup(int device)
+{
+    if (device < 0 || device >= NA_MAX_DEVICES) return fail(GALN_HIP_E_DEVICE, "device %d beyond %d", device, NA_MAX_DEVICES);
+    std::lock_guard<std::mutex> lk(g_na_setup_mu);
+    if (g_na_ptr[device]) return 0;
+    void *p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) {
+        (void)hipGetLastError();
+        return fail(GALN_HIP_E_DEVICE, "scratch array of device %d: hipGetSymbolAddress: %s", device,
+                    hipGetErrorString(e));
+    }
+    g_na_ptr[device] = static_cast<uint8_t *>(p);
+    return 0;
+}
+
+// Longest read (capped at MAXLEN + 1) and the number of reads whose batch index is out of range.
+__global__ void k_na_scan(const int32_t *qlen, const int32_t *batch, int n, int n_batches, int32_t *res)
+{
+    int mx = 0, bad = 0;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        mx = max(mx, min(qlen[i], MAXLEN + 1));
+        bad += (batch[i] < 0 || batch[i] >= n_batches) ? 1 : 0;
+    }
+    if (mx > 0) atomicMax(res, mx);
+    if (bad) atomicAdd(res + 1, bad);
+}
+
+// Offsets inside the array.
+struct NaLayout {
+    size_t ctr, top, algo, scan, ghead, gbump, bopts, nstacks, fail0, fail1, pops, lanes;
+};
+
+NaLayout na_layout()
+{
+    NaLayout L{};
+    L.ctr = 0;      // 8 x i32: [0],[1] failure counts, [2] claim counter, [3] unused, [4..6] failure kinds
+    L.top = 64;     // u64 records reserved in the caller's array
+    L.algo = 72;    // u64 algorithmic index bytes
+    L.scan = 80;    // 2 x i32 (k_na_scan)
+    L.ghead = 256;  // NSHARD x u64 free-granule stack heads
+    L.gbump = L.ghead + NSHARD * 8;
+    size_t off = align_up(L.gbump + 4, 256);
+    L.bopts = off;
+    off = align_up(off + (size_t)NA_MAX_BATCHES * sizeof(galn_search_opt_t), 256);
+    L.nstacks = off;
+    off = align_up(off + (size_t)NA_MAX_BATCHES * 4, 256);
+    L.fail0 = off;
+    off += (size_t)NA_MAX_READS * 4;
+    L.fail1 = off;
+    off += (size_t)NA_MAX_READS * 4;
+    L.pops = off;  // pop counts when the caller does not ask for them
+    off += (size_t)NA_MAX_READS * 8;
+    L.lanes = align_up(off, 256);
+    return L;
+}
+
+// Bytes of one pool granule with its chunk links and free-stack link.
+constexpr size_t NA_GRAN_BYTES = (size_t)GRAN * CH * sizeof(Ent) + (size_t)GRAN * 4 + 4;
+
+// Lanes that fit the array beside the pool with their region of `slot_bytes`: at most `want`,
+// each with at least two granules of pool. Returns 0 if not even one fits.
+int na_lanes(size_t slot_bytes, int want, uint64_t *n_gran)
+{
+    const size_t avail = NA_SCRATCH_BYTES - na_layout().lanes - 3 * 256;
+    for (int lanes = std::max(1, want);; lanes = lanes > 1 ? lanes / 2 : 0) {
+        if (lanes == 0) return 0;
+        const size_t region = align_up((size_t)lanes * slot_bytes, 256);
+        if (region >= avail) continue;
+        const uint64_t g = std::min<uint64_t>((avail - region) / NA_GRAN_BYTES, MAX_POOL_CHUNKS / GRAN);
+        if (g >= 2ull * (uint64_t)lanes) {
+            *n_gran = g;
+            return lanes;
+        }
+    }
+}
+
+int noalloc_search(galn_hip_index *idx, const galn_hip_reads *reads, galn_hip_out *out, uint32_t flags)
+{
+    if (!idx || !reads || !out) return fail(GALN_HIP_E_ARG, "search_device_noalloc: null argument");
+    const int n = reads->n_reads;
+    if (n < 0 || n > NA_MAX_READS)
+        return fail(GALN_HIP_E_ARG, "search_device_noalloc: %d reads (at most %d per call)", n, NA_MAX_READS);
+    if (reads->n_batches < 1 || reads->n_batches > NA_MAX_BATCHES || !reads->bopts || !reads->n_stacks)
+        return fail(GALN_HIP_E_ARG, "search_device_noalloc: %d batches (1 to %d, with host option arrays)",
+                    reads->n_batches, NA_MAX_BATCHES);
+    for (int b = 0; b < reads->n_batches; ++b)
+        if (reads->bopts[b].seed_len < 0) return fail(GALN_HIP_E_UNSUPPORTED, "reads: negative seed length");
+    const bool trace = (flags & GALN_HIP_TRACE) != 0;
+    if (n > 0 && (!reads->query || !reads->qoff || !reads->qlen || !reads->max_diff || !reads->batch ||
+                  !out->n_aln || !out->rec_off || (out->rec_cap > 0 && !out->rec) || (trace && !out->trace)))
+        return fail(GALN_HIP_E_ARG, "search_device_noalloc: null array");
+    int r = busy_check(idx, "search_device_noalloc");
+    if (r) return r;
+    HIP_OK(hipSetDevice(idx->device));
+    if ((r = noalloc_setup(idx->device))) return r;
+    std::lock_guard<std::mutex> lk(g_na_mu[idx->device]);
+    out->n_rec = 0;
+    if (n == 0) return 0;
+
+    uint8_t *const S = g_na_ptr[idx->device];
+    const NaLayout NL = na_layout();
+    int32_t *const ctr = reinterpret_cast<int32_t *>(S + NL.ctr);
+    unsigned long long *const top = reinterpret_cast<unsigned long long *>(S + NL.top);
+    unsigned long long *const algo = reinterpret_cast<unsigned long long *>(S + NL.algo);
+    int32_t *const scan = reinterpret_cast<int32_t *>(S + NL.scan);
+    unsigned long long *const ghead = reinterpret_cast<unsigned long long *>(S + NL.ghead);
+    uint32_t *const gbump = reinterpret_cast<uint32_t *>(S + NL.gbump);
+    galn_search_opt_t *const d_bopts = reinterpret_cast<galn_search_opt_t *>(S + NL.bopts);
+    int32_t *const d_nstacks = reinterpret_cast<int32_t *>(S + NL.nstacks);
+    int32_t *const d_fail[2] = {reinterpret_cast<int32_t *>(S + NL.fail0), reinterpret_cast<int32_t *>(S + NL.fail1)};
+    uint64_t *const d_pops = out->n_pops ? out->n_pops : reinterpret_cast<uint64_t *>(S + NL.pops);
+    hipStream_t st = idx->stream;
+
+    // per-batch options in, counters cleared (fill kernels: the array keeps the last call's contents)
+    const int nb = reads->n_batches;
+    HIP_OK(hipMemcpyAsync(d_bopts, reads->bopts, (size_t)nb * sizeof(galn_search_opt_t), hipMemcpyHostToDevice, st));
+    HIP_OK(hipMemcpyAsync(d_nstacks, reads->n_stacks, (size_t)nb * 4, hipMemcpyHostToDevice, st));
+    FILL(S, 0, 256);  // counters, record top, byte count, scan results
+    FILL(out->rec_off, 0, (size_t)n * 8);
+    hipLaunchKernelGGL(k_na_scan, dim3(std::min(1024, (n + 255) / 256)), dim3(256), 0, st, reads->qlen, reads->batch, n,
+                       nb, scan);
+    HIP_OK(hipGetLastError());
+    int32_t hs[2] = {0, 0};
+    HIP_OK(hipMemcpyAsync(hs, scan, sizeof hs, hipMemcpyDeviceToHost, st));
+    HIP_OK(hipStreamSynchronize(st));
+    if (hs[1] > 0) return fail(GALN_HIP_E_ARG, "reads: %d batch indexes out of range", hs[1]);
+
+    // what readset_fill derives from the reads
+    const int max_len = std::max(1, std::min(hs[0], MAXLEN));
+    int seed_len = 0, max_stacks = 1, max_entries = 0;
+    bool nonneg = true;
+    for (int b = 0; b < nb; ++b) {
+        const galn_search_opt_t &o = reads->bopts[b];
+        if (o.seed_len < max_len) seed_len = std::max(seed_len, o.seed_len);
+        max_stacks = std::max(max_stacks, reads->n_stacks[b]);
+        max_entries = std::max(max_entries, std::max(o.max_entries, 0));
+        nonneg = nonneg && o.s_mm >= 0 && o.s_gapo >= 0 && o.s_gape >= 0;
+    }
+    const int buckets = std::max(1, std::min(max_stacks, MAXB));
+    const int block = buckets * 128 * 4 <= 65536 ? 128 : 64;
+    const int own_full = (int)(((int64_t)(max_entries + 16) / CH + MAXB + 2) / GRAN + 2);
+
+    KArgs a{};
+    a.bwt = idx->d_bwt;
+    a.meta = idx->meta;
+    a.query = reads->query;
+    a.qoff = reads->qoff;
+    a.qlen = reads->qlen;
+    a.maxdiff = reads->max_diff;
+    a.batch = reads->batch;
+    a.bopts = d_bopts;
+    a.nstacks = d_nstacks;
+    a.qnext = ctr + 2;
+    a.fail_kind = ctr + 4;
+    a.algo_bytes = algo;
+    a.out_n = out->n_aln;
+    a.out_off = out->rec_off;
+    a.out_pops = d_pops;
+    a.out_trace = out->trace;
+    a.out_bytes = out->occ_bytes;
+    a.out_rec = out->rec;
+    a.out_cap = out->rec_cap;
+    a.out_top = top;
+    a.ghead = ghead;
+    a.gbump = gbump;
+    a.fuse = ((flags & GALN_HIP_NOFUSE) || !nonneg) ? 0 : 1;
+    a.heavy_n = ctr + 3;  // no pop budget: nothing is handed on
+
+    int rec_cap = 64, want = std::min(n, 4096), fi = 0;
+    int32_t n_todo = n;
+    bool out_full = false;
+    for (int round = 0; n_todo > 0; ++round) {
+        if (round >= 24) return fail(GALN_HIP_E_NOMEM, "search_device_noalloc: reads still failing after %d re-runs", round);
+        const Layout L0 = make_layout(max_len, seed_len, rec_cap, own_full);
+        uint64_t n_gran = 0;
+        const int lanes = na_lanes(L0.slot_bytes, std::min(want, (int)n_todo), &n_gran);
+        if (lanes == 0)
+            return fail(GALN_HIP_E_UNSUPPORTED, "search_device_noalloc: a lane's %llu bytes and its pool do not fit in %zu bytes",
+                        (unsigned long long)L0.slot_bytes, NA_SCRATCH_BYTES);
+        const int own_cap = (int)std::min<uint64_t>((uint64_t)own_full, n_gran);
+        const Layout L = make_layout(max_len, seed_len, rec_cap, own_cap);
+        uint8_t *const lanes_base = S + NL.lanes;
+        uint8_t *const pool = S + align_up(NL.lanes + (size_t)lanes * L.slot_bytes, 256);
+        a.scratch = lanes_base;
+        a.slot_bytes = L.slot_bytes;
+        a.sw_off = L.sw_off;
+        a.rec_off = L.rec_off;
+        a.own_off = L.own_off;
+        a.own_cap = L.own_cap;
+        a.keep = 1;
+        a.rec_cap = rec_cap;
+        a.pool_chk = reinterpret_cast<Ent *>(pool);
+        a.pool_link = reinterpret_cast<uint32_t *>(pool + align_up(n_gran * GRAN * CH * sizeof(Ent), 256));
+        a.gnext = reinterpret_cast<uint32_t *>(reinterpret_cast<uint8_t *>(a.pool_link) + align_up(n_gran * GRAN * 4, 256));
+        a.n_gran = (uint32_t)n_gran;
+        a.list = round == 0 ? nullptr : d_fail[fi];
+        a.list_base = 0;
+        a.n_list = n_todo;
+        a.n_slots = lanes;
+        const int fo = round == 0 ? 0 : fi ^ 1;
+        a.fail_list = d_fail[fo];
+        a.fail_n = ctr + fo;
+        FILL(ctr + fo, 0, sizeof(int32_t));
+        FILL(ctr + 2, 0, 2 * sizeof(int32_t));
+        FILL(ctr + 4, 0, 3 * sizeof(int32_t));
+        FILL(ghead, 0xffffffffu, NSHARD * 8);
+        FILL(gbump, 0, 4);
+        r = launch(idx, a, block, (lanes + block - 1) / block, true, trace, buckets);
+        if (r) return r;
+        int32_t c[8];
+        HIP_OK(hipMemcpyAsync(c, ctr, sizeof c, hipMemcpyDeviceToHost, st));
+        HIP_OK(hipStreamSynchronize(st));
+        if (c[4]) {  // the caller's record array is full: stop, and report what was reserved
+            out_full = true;
+            break;
+        }
+        if (c[5]) rec_cap *= 16;
+        if (c[6]) want = std::max(1, lanes / 4);
+        fi = fo;
+        n_todo = c[fo];
+    }
+    unsigned long long t = 0;
+    HIP_OK(hipMemcpyAsync(&t, top, sizeof t, hipMemcpyDeviceToHost, st));
+    HIP_OK(hipStreamSynchronize(st));
+    out->n_rec = t;
+    if (out_full)
+        return fail(GALN_HIP_E_RECCAP, "search_device_noalloc: record array holds %llu; at least %llu needed",
+                    (unsigned long long)out->rec_cap, t);
+    return 0;
+}
+
+}  // namespace
+
+extern "C" int galn_hip_search_device_noalloc(galn_hip_index *idx, const galn_hip_reads *reads, galn_hip_out *out,
+                                              uint32_t flags)
+{
+    return noalloc_search(idx, reads, out, flags);
+}

@@ -31,6 +31,7 @@ HIP_SYMBOLS = (
     "galn_hip_readset_reload", "galn_hip_readset_autotune", "galn_hip_readset_kernel_ms", "galn_hip_readset_launches",
     "galn_hip_readset_engine_stats", "galn_hip_server_open", "galn_hip_server_submit", "galn_hip_server_wait",
     "galn_hip_server_result", "galn_hip_server_stats", "galn_hip_server_close", "galn_hip_gather_ceiling",
+    "galn_hip_search_device_noalloc",
 )
 HOST_SYMBOLS = (
     "galn_gap_opt_default", "galn_parse_aln_args", "galn_cal_maxdiff", "galn_batch_opt", "galn_read_maxdiff",
@@ -130,6 +131,7 @@ def hip_lib():
         L.galn_hip_readset_engine_stats.argtypes = [P, P]
         L.galn_hip_search_plain.argtypes = [P, C.POINTER(Reads), C.POINTER(Out), C.c_uint32]
         L.galn_hip_search_batched.argtypes = [P, C.POINTER(Reads), C.POINTER(Out), C.c_uint32, C.POINTER(Tune)]
+        L.galn_hip_search_device_noalloc.argtypes = [P, C.POINTER(Reads), C.POINTER(Out), C.c_uint32]
         L.galn_hip_get_limits.argtypes = [C.POINTER(Limits)]
         L.galn_hip_gather_ceiling.argtypes = [C.c_int, C.c_uint64, C.c_int, C.c_int, C.c_int,
                                               C.POINTER(C.c_double), C.POINTER(C.c_double)]

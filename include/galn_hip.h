@@ -166,6 +166,24 @@ int galn_hip_search_plain(galn_hip_index *idx, const galn_hip_reads *reads, galn
 int galn_hip_search_batched(galn_hip_index *idx, const galn_hip_reads *reads, galn_hip_out *out,
                             uint32_t flags, const galn_hip_tune_t *tune);
 
+/* Search that allocates nothing. No counterpart in the reference: it is for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc). Same records, pop counts and trace hashes as galn_hip_search_plain, from
+ * the thread-per-read engine with every read claimed from a device counter.
+ *   reads:  query, qoff, qlen, max_diff, batch in DEVICE memory; bopts and n_stacks in host memory.
+ *   out:    n_aln, rec_off, rec, and the optional n_pops, trace, occ_bytes in DEVICE memory.
+ *           Read i's records are rec[rec_off[i] .. rec_off[i] + n_aln[i]) (each read's records are
+ *           contiguous; reads follow each other in no set order; rec_off is 0 for reads without
+ *           records). out->n_rec (host) = records written; with GALN_HIP_E_RECCAP, at least that
+ *           many are needed.
+ * Everything else lives in one static device array of 256 MiB per GPU, held from the library's
+ * first use on that GPU until the process ends: at most 1,048,576 reads and 4096 batches per call
+ * (beyond: GALN_HIP_E_ARG); reads whose stacks outgrow the array's pool are re-run with fewer
+ * lanes sharing it. Calls on one GPU share its array and take turns (a lock per device); the
+ * array keeps the last call's contents. Synchronous: returns after the outputs are written. */
+int galn_hip_search_device_noalloc(galn_hip_index *idx, const galn_hip_reads *reads, galn_hip_out *out,
+                                   uint32_t flags);
+
 /* Device-resident reads: upload once, search many times (benchmarks, serving). */
 int galn_hip_readset_upload(galn_hip_index *idx, const galn_hip_reads *reads, galn_hip_readset **out);
 int galn_hip_readset_free(galn_hip_readset *rs);
