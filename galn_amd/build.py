@@ -1,6 +1,6 @@
 """Builds the native parts in-tree:
 
-  galn_amd/lib/libgaln_hip.so   HIP search kernels + C ABI + serving loop (hipcc, gfx950)
+  galn_amd/lib/libgaln_hip.so   HIP search kernels + C ABI + serving loop + .bwt builder (hipcc, gfx950)
   galn_amd/lib/libgaln_host.so  host side of `galn aln` and the CPU search engine (g++), links libgaln_hip
   galn_amd/bin/galn             the `galn` executable
   oracle/liboracle.so           the CPU oracle (test infrastructure), and oracle/_ref/ when
@@ -46,7 +46,8 @@ def build_hip() -> str:
     out = os.path.join(LIB, "libgaln_hip.so")
     tmp = out + ".tmp"
     _run([HIPCC, *HIP_FLAGS, "-shared", f"-I{INC}", "-o", tmp, os.path.join(CSRC, "galn_search.hip"),
-          os.path.join(CSRC, "galn_membench.hip"), os.path.join(CSRC, "galn_server.cpp"), "-lpthread"],
+          os.path.join(CSRC, "galn_membench.hip"), os.path.join(CSRC, "galn_index.hip"), os.path.join(CSRC, "galn_server.cpp"),
+          "-lpthread"],
          quiet_feature_warning=True)
     os.replace(tmp, out)
     return out

@@ -7,9 +7,11 @@ T = F . revcomp(F) (F = all contigs concatenated, ambiguous bases replaced by
 (src/index/cpu_ref.c:56-106, src/index/idxcommon.c:128-184, upstream
 bntseq.c bns_fasta2bntseq): any correct suffix array gives the same BWT.
 
-The suffix array is built by prefix doubling over torch tensors, so the same
-code runs on the CPU (tests) and on the GPU (large synthetic references):
-sort key = (rank[i], rank[i+h]) until all ranks are distinct.
+On a GPU the HIP builder of libgaln_hip.so (galn_hip_build_bwt: bucketed
+radix sorts of 27-symbol keys with refinement, any length that fits HBM) runs.
+On the CPU the suffix array is built by prefix doubling over torch tensors
+(sort key = (rank[i], rank[i+h]) until all ranks are distinct), for the small
+texts of the CPU tests.
 """
 from __future__ import annotations
 
@@ -172,9 +174,31 @@ def read_bwt(path: str) -> tuple[dict, np.ndarray]:
     return meta, words
 
 
-def build_index(seq: bytes | np.ndarray, prefix: str, device: str | torch.device = "cpu") -> dict:
-    """Build ``<prefix>.bwt`` from ASCII reference bases; returns the descriptor."""
+def write_sa(path: str, primary: int, L2: list[int], n: int, samples: np.ndarray) -> None:
+    """bwt_dump_sa format (src/index/idxcommon.c:137-153): primary, L2[1..4], the interval 32,
+    n, then the suffix at rows 32, 64, ... (all u64)."""
+    with open(path, "wb") as fh:
+        fh.write(struct.pack("<7Q", primary, *L2[1:5], 32, n))
+        fh.write(np.ascontiguousarray(samples, dtype=np.uint64).tobytes())
+
+
+def build_index(seq: bytes | np.ndarray, prefix: str, device: str | torch.device = "cpu", want_sa: bool = False) -> dict:
+    """Build ``<prefix>.bwt`` (and with `want_sa` ``<prefix>.sa``) from ASCII reference bases;
+    returns the descriptor. On a GPU device the HIP builder (galn_hip_build_bwt) runs; on the
+    CPU the prefix doubling above (small texts)."""
     f2 = encode_forward(seq)
+    dev = torch.device(device)
+    if dev.type == "cuda":
+        from . import native as nv
+        m, words, sa, _ = nv.build_bwt(f2, dev.index or 0, want_sa=want_sa)
+        L2 = [int(m.L2[i]) for i in range(5)]
+        primary = int(m.primary)
+        write_bwt(prefix + ".bwt", primary, L2, words)
+        if want_sa:
+            write_sa(prefix + ".sa", primary, L2, int(m.seq_len), sa)
+        return {"primary": primary, "L2": L2, "seq_len": L2[4], "bwt_size": int(words.size)}
+    if want_sa:
+        raise ValueError("build_index: SA samples come from the GPU builder only")
     primary, L = bwt_from_forward(f2, device=device)
     L2, words = pack_bwt(L)
     write_bwt(prefix + ".bwt", primary, L2, words)

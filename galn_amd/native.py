@@ -31,7 +31,7 @@ HIP_SYMBOLS = (
     "galn_hip_readset_reload", "galn_hip_readset_autotune", "galn_hip_readset_kernel_ms", "galn_hip_readset_launches",
     "galn_hip_readset_engine_stats", "galn_hip_server_open", "galn_hip_server_submit", "galn_hip_server_wait",
     "galn_hip_server_result", "galn_hip_server_stats", "galn_hip_server_close", "galn_hip_gather_ceiling",
-    "galn_hip_search_device_noalloc",
+    "galn_hip_search_device_noalloc", "galn_hip_bwt_words", "galn_hip_build_bwt",
 )
 HOST_SYMBOLS = (
     "galn_gap_opt_default", "galn_parse_aln_args", "galn_cal_maxdiff", "galn_batch_opt", "galn_read_maxdiff",
@@ -132,6 +132,9 @@ def hip_lib():
         L.galn_hip_search_plain.argtypes = [P, C.POINTER(Reads), C.POINTER(Out), C.c_uint32]
         L.galn_hip_search_batched.argtypes = [P, C.POINTER(Reads), C.POINTER(Out), C.c_uint32, C.POINTER(Tune)]
         L.galn_hip_search_device_noalloc.argtypes = [P, C.POINTER(Reads), C.POINTER(Out), C.c_uint32]
+        L.galn_hip_bwt_words.restype = C.c_uint64
+        L.galn_hip_bwt_words.argtypes = [C.c_uint64]
+        L.galn_hip_build_bwt.argtypes = [P, C.c_uint64, C.c_int, C.POINTER(BwtMeta), P, P, P]
         L.galn_hip_get_limits.argtypes = [C.POINTER(Limits)]
         L.galn_hip_gather_ceiling.argtypes = [C.c_int, C.c_uint64, C.c_int, C.c_int, C.c_int,
                                               C.POINTER(C.c_double), C.POINTER(C.c_double)]
@@ -196,6 +199,25 @@ def limits() -> dict:
     lim = Limits()
     hip_lib().galn_hip_get_limits(C.byref(lim))
     return {f: getattr(lim, f) for f, _ in Limits._fields_}
+
+
+def build_bwt(fwd2: np.ndarray, device: int = 0, want_sa: bool = False) -> tuple[BwtMeta, np.ndarray, np.ndarray | None, dict]:
+    """galn_hip_build_bwt: the .bwt body of T$ (T = F . revcomp(F)) for forward 2-bit codes,
+    built on `device`; with `want_sa` also the SA samples of rows 32, 64, ... Returns (meta,
+    body words, samples or None, {chunks, rounds, largest_chunk})."""
+    f = np.ascontiguousarray(fwd2, dtype=np.uint8)
+    n = 2 * int(f.size)
+    L = hip_lib()
+    words = np.empty(int(L.galn_hip_bwt_words(n)), np.uint32)
+    sa = np.empty(max((n + 32) // 32 - 1, 1), np.uint64) if want_sa else None
+    st = np.zeros(4, np.float64)
+    m = BwtMeta()
+    rc = L.galn_hip_build_bwt(_p(f), int(f.size), int(device), C.byref(m), _p(words), _p(sa) if sa is not None else None,
+                              _p(st))
+    if rc != 0:
+        raise GalnError(f"build_bwt failed ({rc}): {last_error()}")
+    info = {"chunks": int(st[0]), "rounds": int(st[1]), "largest_chunk": int(st[2])}
+    return m, words, (sa[: (n + 32) // 32 - 1] if sa is not None else None), info
 
 
 # --------------------------------------------------------------------------

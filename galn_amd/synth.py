@@ -27,9 +27,14 @@ for _i, _c in enumerate(b"ACGT"):
     _NT4[_c + 32] = _i
 
 
-def random_seq(rng: np.random.Generator, n: int, gc: float = 0.5) -> np.ndarray:
+def random_seq(rng: np.random.Generator, n: int, gc: float = 0.5, step: int = 1 << 26) -> np.ndarray:
+    """n bases with the given GC share. Drawn in pieces of `step` (choice with p takes one
+    uniform double per base, in order, so the pieces give the same bases as one call)."""
     p = np.array([(1 - gc) / 2, gc / 2, gc / 2, (1 - gc) / 2])
-    return ACGT[rng.choice(4, size=n, p=p).astype(np.uint8)]
+    out = np.empty(n, dtype=np.uint8)
+    for s in range(0, n, step):
+        out[s:s + step] = ACGT[rng.choice(4, size=min(step, n - s), p=p).astype(np.uint8)]
+    return out
 
 
 def mutate_subs(rng: np.random.Generator, s: np.ndarray, rate: float) -> np.ndarray:
@@ -280,7 +285,8 @@ def human_like_reference(seed: int, length: int, n_contigs: int = 24, gc: float 
         dst = int(rng.integers(0, length - rep.size))
         s[dst:dst + rep.size] = mutate_subs(rng, rep, 0.01)
         done += rep.size
-    cuts = np.sort(rng.choice(np.arange(1, length), size=n_contigs - 1, replace=False))
+    # cut points 1 .. length-1 (choice over the count draws the same indexes as over the array)
+    cuts = np.sort(1 + rng.choice(length - 1, size=n_contigs - 1, replace=False))
     parts = np.split(s, cuts)
     return [(f"chr{i + 1}", p) for i, p in enumerate(parts)]
 

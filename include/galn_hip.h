@@ -238,6 +238,26 @@ int galn_hip_server_stats(galn_hip_server *srv, double out[8]);
 /* Finishes the requests already submitted, then stops the worker. */
 int galn_hip_server_close(galn_hip_server *srv);
 
+/* u32 words of a .bwt body for a BWT of n symbols: 8 * (ceil(n/128) + 1) + ceil(n/16)
+ * (galn_bwt_words, src/index/idxcommon.h:40-44). */
+uint64_t galn_hip_bwt_words(uint64_t n);
+
+/* The .bwt of T$ with T = F . revcomp(F), built on `device` (replaces the index builders of
+ * `galn index`: src/index/cpu_ref.c:56-106, src/index/gpu_sa.h:12-20 galn_gpu_bwt,
+ * src/index/gpu_index.cu:211-408).
+ *   fwd:  the forward reference, `len` 2-bit codes (0..3; ambiguous bases already replaced by
+ *         lrand48() & 3 as bns_fasta2bntseq does). The BWT has n = 2 * len symbols; any n that
+ *         fits the device (n/4 bytes of text, n/4 of BWT, n/2 of output, ~100 bytes per suffix of
+ *         one chunk of at most 2^31 suffixes) is supported, 2^32 and beyond included.
+ *   bwt:  caller-owned, galn_hip_bwt_words(n) words: the body of the .bwt file (what follows its
+ *         40-byte header: primary, L2[1..4]). meta gets primary, L2, seq_len = n, bwt_size.
+ *   sa_samples: optional, (n + 32) / 32 - 1 words: the suffix at rows 32, 64, ... (the .sa samples
+ *         of bwt_cal_sa(bwt, 32)).
+ *   stats: optional [4]: chunks, most refinement rounds of a chunk, largest chunk, 0.
+ * Byte-identical to the reference builder's .bwt. */
+int galn_hip_build_bwt(const uint8_t *fwd, uint64_t len, int device, galn_bwt_meta_t *meta, uint32_t *bwt,
+                       uint64_t *sa_samples, double *stats);
+
 /* Random-access ceiling of `device`: pointer chase over a `ws_bytes` buffer of
  * `line_bytes` (64 or 128) lines in one pseudo-random cycle, waves_per_cu * CUs
  * independent chains of `iters` dependent whole-line loads each. Returns the

@@ -78,6 +78,12 @@ SPECS = {
 }
 
 
+# genome-scale texts (no reads): the index builder's parity above 2^32 symbols
+LARGE_SPECS = {
+    "copies_4g": {"ref": ("copies", 91, 275_000_000, 8)},
+}
+
+
 def opts_and_files(v) -> tuple[list[str], list[int] | None]:
     """An option list of a spec, and the read files it runs on (None: all)."""
     return (list(v), None) if isinstance(v, list) else (list(v[0]), list(v[1]))
