@@ -35,6 +35,11 @@ SPECS = {
         "ref": ("plain", 11, 500_000), "reads": ("plain", 12, 3000, 100),
         "opts": {"default": []},
     },
+    # the plain small configuration at full size: 5 Mbp i.i.d. reference, 100k x 100 bp reads
+    "plain_full": {
+        "ref": ("plain", 101, 5_000_000), "reads": ("plain", 102, 100_000, 100),
+        "opts": {"default": []},
+    },
     "adna_mixed": {
         "ref": ("adna", 21, 2_000_000), "reads": ("adna", 22, 8000),
         "opts": {"adna": ADNA},

@@ -21,7 +21,7 @@ from galn_amd import native as nv
 pytestmark = pytest.mark.gpu
 
 # fixtures with distinct references
-FIXTURE_REFS = ["plain_small", "adna_mixed", "human_like", "deep_repeats", "fastq_edge", "beyond_limits"]
+FIXTURE_REFS = ["plain_small", "plain_full", "adna_mixed", "human_like", "deep_repeats", "fastq_edge", "beyond_limits"]
 
 
 def _seq(name):
