@@ -17,6 +17,12 @@ reads and checks the GPU records against it, and reports the dominant kernel's
 achieved algorithmic bytes/s against MI355X's HBM peak and the measured
 random-access (pointer-chase) ceiling.
 
+Other workloads (--workload): `plain` (5 Mbp i.i.d. reference, 100k x 100 bp reads, default
+options), `human` (3.1 Gbp human-like reference, 50M reads of 30-150 bp, aDNA options, and the
+serving sweep on that resident index with requests of 30-250 bp reads), `deep` (20 Gbp: a 2.5 Gbp
+genome and 8 diverged copies, n = 4e10 > 2^32; 2 x 20M x 150 bp pairs, aln -n 0.04 -o 2 -e 6 -k 2).
+Their indexes are built on the GPU by the library's builder.
+
 Prints one JSON line on stdout; progress goes to stderr.
 """
 from __future__ import annotations
@@ -38,6 +44,17 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 METRIC = "reads/s aligned, bit-exact .sai, 1 GPU, batched vs plain; p50/p99 request ms"
+WORKLOADS = {
+    "adna": "batched aDNA mixed length: 100 Mbp reference (41% GC, 10% duplicated segments, 1% N), "
+            "10M reads 30-150 bp (median 45), aln -l 1024 -n 0.01 -o 2",
+    "plain": "plain small: 5 Mbp i.i.d. reference, 100k x 100 bp reads (1% substitutions, 0.1% indels), "
+             "default aln options",
+    "human": "human scale: 3.1 Gbp reference (24 contigs, 41% GC, 45% interspersed repeats, tandem repeats), "
+             "50M reads 30-150 bp with deamination, aln -l 1024 -n 0.01 -o 2; serving on the same resident index "
+             "with requests of 1k-1M reads x 30-250 bp, 50/50 default / aDNA options",
+    "deep": "deep repeats: 20 Gbp (a 2.5 Gbp genome and 8 copies at 1-3% substitutions, 0.2% indels; n = 4e10), "
+            "2 x 20M x 150 bp pairs (both mates' files as one read set), aln -n 0.04 -o 2 -e 6 -k 2",
+}
 HBM_PEAK_GBS = 8000.0
 
 
@@ -135,7 +152,8 @@ def measured_traffic(kernel: str, args, tuned: dict) -> int | None:
     """HBM bytes per launch of `kernel` measured by the FETCH_SIZE / WRITE_SIZE passes recorded
     in profiles/ (scaled by a device copy of known size): for the default workload run with the
     tuning that was profiled, else None."""
-    if args.reads != 10_000_000 or args.ref_bp != 100_000_000 or not os.path.exists(TRAFFIC_PROFILE):
+    if (args.workload != "adna" or args.reads != 10_000_000 or args.ref_bp != 100_000_000
+            or not os.path.exists(TRAFFIC_PROFILE)):
         return None
     value, config = None, None
     for ln in open(TRAFFIC_PROFILE):
@@ -244,22 +262,28 @@ def main() -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--reads", type=int, default=10_000_000)
-    ap.add_argument("--ref-bp", type=int, default=100_000_000)
+    ap.add_argument("--workload", choices=list(WORKLOADS), default="adna")
+    ap.add_argument("--reads", type=int, default=0, help="reads (pairs for `deep`); 0: the workload's size")
+    ap.add_argument("--ref-bp", type=int, default=0, help="reference bp (base genome for `deep`); 0: the workload's")
     ap.add_argument("--cpu-sample", type=int, default=0, help="reads for the CPU baseline (0: ~15 s of CPU work)")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--plain-steps", type=int, default=1, help="timed steps of the plain form")
-    ap.add_argument("--plain-reads", type=int, default=1_000_000,
-                    help="reads the plain form is timed on, the first ones of the workload (0: all; it is ~60x slower)")
+    ap.add_argument("--plain-reads", type=int, default=-1,
+                    help="reads the plain form is timed on, the first ones of the workload (0: all; it is ~60x "
+                         "slower; -1: 1M, or 200k for the genome-scale workloads)")
     ap.add_argument("--serve-seconds", type=float, default=8.0, help="arrival time per offered load (0: no serving)")
     ap.add_argument("--serve-max-request", type=int, default=1_000_000)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="also copy a buffer of this many MiB device to device three times (the known-size "
                          "kernel that scales the FETCH_SIZE / WRITE_SIZE counters)")
+    ap.add_argument("--autotune-reads", type=int, default=0,
+                    help="reads the autotuner searches (the first ones; 0: all for adna/plain, 200k otherwise)")
     ap.add_argument("--workdir", default="")
     args = ap.parse_args()
+    # the library reports autotune trials and index-build chunks on stderr (long, otherwise silent calls)
+    os.environ.setdefault("GALN_HIP_VERBOSE", "1")
     if args.gpus != 1:
         sys.stderr.write("bench.py: measures one GPU only (reads shard across GPUs by read, each GPU holding "
                          "the whole index, with no exchange during the search); run with --gpus 1\n")
@@ -268,7 +292,7 @@ def main() -> int:
     import torch
 
     from galn_amd import native as nv
-    from galn_amd import workloads
+    from galn_amd import synth, workloads
 
     if not torch.cuda.is_available():
         sys.stderr.write("bench.py: no GPU visible\n")
@@ -277,9 +301,30 @@ def main() -> int:
     os.makedirs(workdir, exist_ok=True)
     threads = args.cpu_threads or cpu_threads()
 
-    log(f"building workload: {args.ref_bp} bp reference, {args.reads} reads (index on the GPU)")
+    # (reference bp, reads timed, reads of the configuration): the genome-scale configurations are
+    # timed on the first reads of their generators (reads are i.i.d. samples of the same model;
+    # their repeat-rich reads backtrack ~70x more than the aDNA workload's)
+    sizes = {"adna": (100_000_000, 10_000_000, 10_000_000), "plain": (5_000_000, 100_000, 100_000),
+             "human": (3_100_000_000, 2_000_000, 50_000_000), "deep": (2_500_000_000, 2_500_000, 20_000_000)}[args.workload]
+    args.ref_bp = args.ref_bp or sizes[0]
+    args.reads = args.reads or sizes[1]
+    log(f"building workload {args.workload}: {args.ref_bp} bp reference, {args.reads} reads (index on the GPU)")
     t = time.time()
-    w = workloads.adna_mixed(workdir, ref_bp=args.ref_bp, n_reads=args.reads, device="cuda", workers=threads)
+    serve_pool = None
+    if args.workload == "adna":
+        w = workloads.adna_mixed(workdir, ref_bp=args.ref_bp, n_reads=args.reads, device="cuda", workers=threads)
+    elif args.workload == "plain":
+        w = workloads.plain_small(workdir, ref_bp=args.ref_bp, n_reads=args.reads, device="cuda")
+    elif args.workload == "human":
+        w = workloads.human_scale(workdir, ref_bp=args.ref_bp, n_reads=args.reads, workers=threads)
+        if args.serve_seconds > 0:
+            # serving requests: reads of 30-250 bp from the same reference
+            sr, sl = synth.uniform_length_reads(303, w.ref, 2_000_000, 30, 250, workers=threads)
+            serve_pool = workloads.Workload("serving_pool", w.prefix, w.meta, w.words, sr, sl,
+                                            list(workloads.ADNA_OPTS)).prepare()
+    else:
+        w = workloads.deep_copies(workdir, base_bp=args.ref_bp, n_pairs=args.reads, workers=threads, progress=log)
+    w.ref = None
     log(f"workload ready in {time.time() - t:.1f} s ({w.timings}); n={w.n}, index {w.words.nbytes / 1e6:.1f} MB")
     torch.cuda.synchronize()
     torch.cuda.empty_cache()
@@ -298,8 +343,23 @@ def main() -> int:
     # the batched form: persistent kernel + device work queue, launches replayed as a captured
     # hipGraph, tuning chosen by the library's autotuner on this read set
     BATCHED_FLAGS = nv.GALN_HIP_GRAPH
-    log("autotuning the batched form on this read set")
-    tuned = rs.autotune(BATCHED_FLAGS)
+    at_n = args.autotune_reads or (w.n if args.workload in ("adna", "plain") else 200_000)
+    at_n = min(at_n, w.n)
+    if at_n < w.n:
+        # tune on the first at_n reads, then search the whole read set with that tuning
+        log(f"autotuning the batched form on the first {at_n} reads")
+        hi_q = int((w.qoff[:at_n] + w.lens[:at_n].astype(np.uint64)).max())
+        lo_q = int(w.qoff[:at_n].min())
+        rs_t = nv.ReadSet(index, w.query[lo_q:hi_q], w.qoff[:at_n] - np.uint64(lo_q), w.lens[:at_n].astype(np.int32),
+                          w.maxdiff[:at_n], w.batch[:at_n], w.bopts, w.nstacks)
+        tuned = rs_t.autotune(BATCHED_FLAGS)
+        rs_t.close()
+    else:
+        log("autotuning the batched form on this read set")
+        tuned = rs.autotune(BATCHED_FLAGS)
+    TUNE = nv.Tune()
+    for k, v in tuned.items():
+        setattr(TUNE, k, int(v))
     log(f"autotuned: {tuned}")
     # host arrays for the records, allocated once in pinned memory and reused by every step (as a
     # caller that searches batch after batch would): the copies then run at full PCIe speed
@@ -310,8 +370,10 @@ def main() -> int:
     bufs = rs.result_buffers(want_pops=True, want_bytes=True, alloc=pinned)
     log("warm-up")
     for _ in range(max(args.warmup, 0)):
-        rs.search(nv.FORM_BATCHED, BATCHED_FLAGS)
+        t0 = time.perf_counter()
+        rs.search(nv.FORM_BATCHED, BATCHED_FLAGS, TUNE)
         rs.results(want_pops=False, out=bufs)
+        log(f"  warm-up search: {(time.perf_counter() - t0) * 1e3:.1f} ms")
 
     def timed(rset, n: int, form: int, steps: int, out):
         times, launches = [], []
@@ -319,7 +381,10 @@ def main() -> int:
         for s in range(steps):
             torch.cuda.synchronize()
             t0 = time.perf_counter()
-            rset.search(form, BATCHED_FLAGS if form == nv.FORM_BATCHED else 0)
+            if form == nv.FORM_BATCHED:
+                rset.search(form, BATCHED_FLAGS, TUNE)
+            else:
+                rset.search(form, 0)
             res = rset.results(want_pops=(s == steps - 1), want_bytes=(s == steps - 1), out=out)
             torch.cuda.synchronize()
             times.append(time.perf_counter() - t0)
@@ -329,7 +394,9 @@ def main() -> int:
 
     log(f"timing batched form, {args.steps} steps")
     tb, lb, res_b = timed(rs, w.n, nv.FORM_BATCHED, args.steps, bufs)
-    n_plain = w.n if args.plain_reads <= 0 else min(args.plain_reads, w.n)
+    if args.plain_reads < 0:
+        args.plain_reads = 200_000 if args.workload in ("human", "deep") else 1_000_000
+    n_plain = w.n if args.plain_reads == 0 else min(args.plain_reads, w.n)
     if not args.no_plain:
         # the plain form (~60x slower) on the first n_plain reads, a sample of the same generator;
         # its records must equal the batched form's for those reads
@@ -384,9 +451,8 @@ def main() -> int:
         "vs_baseline": None,
         "dtype": "int64",
         "data": "synthetic (seeded generator; index built on the GPU from the generated reference)",
-        "config": {"workload": "batched aDNA mixed length: 100 Mbp reference (41% GC, 10% duplicated segments, 1% N), "
-                               "10M reads 30-150 bp (median 45), aln -l 1024 -n 0.01 -o 2",
-                   "ref_bp": args.ref_bp, "reads": w.n, "aln_opts": " ".join(w.opts), "form": "batched",
+        "config": {"workload": WORKLOADS[args.workload], "name": args.workload,
+                   "ref_bp": args.ref_bp, "reads": w.n, "config_reads": sizes[2] * (2 if args.workload == "deep" else 1), "aln_opts": " ".join(w.opts), "form": "batched",
                    "autotuned": tuned,
                    "index_bytes": ws},
         "plain": {"value": n_plain / (sum(tp) / len(tp)) if tp else None,
@@ -407,6 +473,9 @@ def main() -> int:
         "ceiling": {"kind": "pointer chase, 64-byte random lines", "ws_bytes": ws, "GB/s": ceiling_gbs,
                     "frac": achieved / ceiling_gbs if ceiling_gbs else None},
     }
+    # the wave engine's counters (rounds, chains, clock ticks per phase) from one more search, not timed
+    rs.search(nv.FORM_BATCHED, BATCHED_FLAGS | nv.GALN_HIP_STATS, TUNE)
+    out["engine_stats"] = rs.engine_stats()
     if args.copy_calibration > 0:
         n = args.copy_calibration << 20
         src = torch.ones(n, dtype=torch.uint8, device="cuda")
@@ -416,9 +485,10 @@ def main() -> int:
         torch.cuda.synchronize()
         log(f"copy calibration: 3 device copies of {n} bytes")
         del src, dst
-    if args.serve_seconds > 0:
-        out["serving"] = serving_sweep(w, index, nv, (0.25, 0.5, 0.75, 0.9), args.serve_seconds,
-                                       min(args.serve_max_request, w.n))
+    if args.serve_seconds > 0 and args.workload in ("adna", "human"):
+        pool = serve_pool if serve_pool is not None else w
+        out["serving"] = serving_sweep(pool, index, nv, (0.25, 0.5, 0.75, 0.9), args.serve_seconds,
+                                       min(args.serve_max_request, pool.n))
     rs.close()
     index.close()
     if not args.no_cpu:

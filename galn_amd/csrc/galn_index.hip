@@ -522,6 +522,8 @@ int build(const uint8_t *fwd, uint64_t len, int device, galn_bwt_meta_t *meta, u
     IHIP(hipMemsetAsync(primary.p, 0xff, 8, st));
     hipLaunchKernelGGL(k_row0, dim3(1), dim3(1), 0, st, Pc, n, R.as<uint64_t>());
     IHIP(hipGetLastError());
+    const char *vb = getenv("GALN_HIP_VERBOSE");
+    const bool verbose = vb && *vb && strcmp(vb, "0") != 0;
     {
         ChunkBufs cb;
         if ((r = chunk_alloc(cb, maxm))) return r;
@@ -539,6 +541,9 @@ int build(const uint8_t *fwd, uint64_t len, int device, galn_bwt_meta_t *meta, u
             int rounds = 0;
             if ((r = sort_chunk(W, cb, Pc, n, c.m, &rounds))) return r;
             max_rounds = std::max(max_rounds, rounds);
+            if (verbose)
+                fprintf(stderr, "[galn_hip] build_bwt: chunk %zu of %zu: %llu suffixes sorted, %d refinement rounds\n",
+                        (size_t)(&c - chunks.data()) + 1, chunks.size(), (unsigned long long)c.m, rounds);
             const uint64_t words = ((c.r0 + c.m - 1) >> 5) - (c.r0 >> 5) + 1;
             hipLaunchKernelGGL(k_emit, dim3(grid_for(words)), dim3(256), 0, st, Pc, cb.V.as<const uint64_t>(), c.m, c.r0,
                                R.as<uint64_t>(), primary.as<unsigned long long>(), samples_out ? S.as<uint64_t>() : nullptr,

@@ -2166,6 +2166,13 @@ int fail(int code, const char *fmt, ...)
     return code;
 }
 
+// progress lines on stderr for long calls (GALN_HIP_VERBOSE set and not "0")
+bool galn_hip_verbose()
+{
+    const char *v = getenv("GALN_HIP_VERBOSE");
+    return v && *v && strcmp(v, "0") != 0;
+}
+
 #define HIP_OK(call)                                                                                   \
     do {                                                                                               \
         hipError_t e_ = (call);                                                                        \
@@ -3255,6 +3262,7 @@ int galn_hip_readset_autotune(galn_hip_index *idx, galn_hip_readset *rs, uint32_
     rs->has_tuned = false;
     galn_hip_tune_t best{};
     best.sort_by_cost = 1;
+    const bool verbose = galn_hip_verbose();
     if (rs->n > 0) {
         double best_ms = 0;
         auto trial = [&](const galn_hip_tune_t &t) -> int {
@@ -3262,13 +3270,19 @@ int galn_hip_readset_autotune(galn_hip_index *idx, galn_hip_readset *rs, uint32_
             const int r = readset_search(rs, 1, flags, &t);
             if (r) return r;
             const double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+            if (verbose)
+                fprintf(stderr, "[galn_hip] autotune: cutoff %d, %d-thread groups, %d waves/CU, %d record slots, "
+                                "%d chunks kept: %.1f ms\n",
+                        t.heavy_pops, t.block_threads, t.waves_per_cu, t.rec_per_lane, t.chunks_per_lane, ms);
             if (best_ms == 0 || ms < best_ms) {
                 best_ms = ms;
                 best = t;
             }
             return 0;
         };
-        // the thread-vs-wave cutoff first (when the wave engine may run), then the workgroup size
+        // the thread-vs-wave cutoff first (when the wave engine may run), then the workgroup size,
+        // then one change at a time from the best so far: the persistent grid's waves per CU, the
+        // record slots per lane, and the stack chunks a lane keeps between reads
         galn_hip_tune_t t{};
         t.sort_by_cost = 1;
         t.block_threads = 128;
@@ -3286,6 +3300,22 @@ int galn_hip_readset_autotune(galn_hip_index *idx, galn_hip_readset *rs, uint32_
         t = best;
         t.block_threads = 64;
         if ((r = trial(t))) return r;
+        const int waves[] = {4, 6}, recs[] = {16, 256}, keeps[] = {GRAN, 16 * GRAN};
+        for (int v : waves) {
+            t = best;
+            t.waves_per_cu = v;
+            if ((r = trial(t))) return r;
+        }
+        for (int v : recs) {
+            t = best;
+            t.rec_per_lane = v;
+            if ((r = trial(t))) return r;
+        }
+        for (int v : keeps) {
+            t = best;
+            t.chunks_per_lane = v;
+            if ((r = trial(t))) return r;
+        }
     }
     rs->tuned = best;
     rs->has_tuned = true;

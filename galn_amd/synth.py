@@ -310,11 +310,17 @@ def _indel_copy(rng, base: np.ndarray, sub_rate: float, indel_rate: float) -> np
     return np.concatenate(out)
 
 
-def diverged_copies_reference(seed: int, base_len: int, copies: int = 8) -> list[tuple[str, np.ndarray]]:
-    """A random base genome and `copies` copies of it, each with 1-3% substitutions and 0.2% indels."""
+def diverged_copies_reference(seed: int, base_len: int, copies: int = 8, progress=None) -> list[tuple[str, np.ndarray]]:
+    """A random base genome and `copies` copies of it, each with 1-3% substitutions and 0.2% indels.
+    `progress(i)` is called after copy i is made."""
     rng = np.random.default_rng(seed)
     base = random_seq(rng, base_len, 0.5)
-    return [(f"copy{i + 1}", _indel_copy(rng, base, float(rng.uniform(0.01, 0.03)), 0.002)) for i in range(copies)]
+    out = []
+    for i in range(copies):
+        out.append((f"copy{i + 1}", _indel_copy(rng, base, float(rng.uniform(0.01, 0.03)), 0.002)))
+        if progress:
+            progress(i)
+    return out
 
 
 def paired_reads(seed: int, ref: np.ndarray, n_pairs: int, length: int = 150, insert_mean: float = 350.0,
@@ -343,13 +349,51 @@ def paired_reads(seed: int, ref: np.ndarray, n_pairs: int, length: int = 150, in
 
 
 def uniform_length_reads(seed: int, ref: np.ndarray, n: int, lo: int = 30, hi: int = 150, sub_rate: float = 0.005,
-                         deam: float = 0.30, random_frac: float = 0.0) -> tuple[np.ndarray, np.ndarray]:
-    """Reads of uniform length in [lo, hi], both strands, deamination and substitutions."""
+                         deam: float = 0.30, random_frac: float = 0.0, workers: int = 1) -> tuple[np.ndarray, np.ndarray]:
+    """Reads of uniform length in [lo, hi], both strands, deamination and substitutions. Chunks
+    are seeded by (seed, chunk), so ``workers`` only changes the speed."""
     rng = np.random.default_rng(seed)
     lens = rng.integers(lo, hi + 1, size=n).astype(np.int64)
     out = np.zeros((n, hi), dtype=np.uint8)
     chunk = 1 << 18
-    for c, s in enumerate(range(0, n, chunk)):
-        m = min(chunk, n - s)
-        out[s:s + m] = _adna_chunk((seed, c, ref, m, lens[s:s + m], random_frac, sub_rate, deam, hi))
+    jobs = [(seed, c, ref, min(chunk, n - s), lens[s:s + chunk], random_frac, sub_rate, deam, hi)
+            for c, s in enumerate(range(0, n, chunk))]
+    starts = list(range(0, n, chunk))
+    if workers > 1 and len(jobs) > 1:
+        from concurrent.futures import ThreadPoolExecutor
+        with ThreadPoolExecutor(workers) as ex:
+            for s, frag in zip(starts, ex.map(_adna_chunk, jobs)):
+                out[s:s + frag.shape[0]] = frag
+    else:
+        for s, job in zip(starts, jobs):
+            out[s:s + job[3]] = _adna_chunk(job)
     return out, lens
+
+
+def _paired_chunk(args):
+    seed, c, ref, m, length, insert_mean, insert_sd, err = args
+    return paired_reads((seed, c), ref, m, length, insert_mean, insert_sd, err)
+
+
+def paired_reads_chunked(seed: int, ref: np.ndarray, n_pairs: int, length: int = 150, insert_mean: float = 350.0,
+                         insert_sd: float = 50.0, err: float = 0.02, chunk: int = 1 << 18,
+                         workers: int = 1) -> tuple[np.ndarray, np.ndarray, np.ndarray, np.ndarray]:
+    """paired_reads for many pairs: chunks seeded by (seed, chunk), generated on `workers` threads."""
+    starts = list(range(0, n_pairs, chunk))
+    jobs = [(seed, c, ref, min(chunk, n_pairs - s), length, insert_mean, insert_sd, err) for c, s in enumerate(starts)]
+    m1 = np.empty((n_pairs, length), np.uint8)
+    m2 = np.empty((n_pairs, length), np.uint8)
+    if workers > 1 and len(jobs) > 1:
+        from concurrent.futures import ThreadPoolExecutor
+        with ThreadPoolExecutor(workers) as ex:
+            parts = ex.map(_paired_chunk, jobs)
+            for s, (a, _, b, _) in zip(starts, parts):
+                m1[s:s + a.shape[0]] = a
+                m2[s:s + b.shape[0]] = b
+    else:
+        for s, job in zip(starts, jobs):
+            a, _, b, _ = _paired_chunk(job)
+            m1[s:s + a.shape[0]] = a
+            m2[s:s + b.shape[0]] = b
+    lens = np.full(n_pairs, length, dtype=np.int64)
+    return m1, lens, m2, lens.copy()

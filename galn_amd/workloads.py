@@ -38,6 +38,7 @@ class Workload:
     nstacks: np.ndarray = field(default=None)
     maxdiff: np.ndarray = field(default=None)
     timings: dict = field(default_factory=dict)
+    ref: np.ndarray = field(default=None)
 
     def prepare(self) -> "Workload":
         t = time.time()
@@ -69,11 +70,20 @@ class Workload:
 
 def _index(ref: np.ndarray, prefix: str, device: str) -> tuple[dict, np.ndarray]:
     """Build <prefix>.bwt, or reuse one an earlier call finished (the name carries the
-    generator's size and seed; <prefix>.done is written after a complete build)."""
+    generator's size and seed; <prefix>.done is written after a complete build). On a GPU the
+    HIP builder's words are returned as built (the file is written for the CLI and the CPU
+    baseline)."""
     done = prefix + ".done"
-    if not (os.path.exists(done) and os.path.exists(prefix + ".bwt")):
-        ix.build_index(ref.tobytes(), prefix, device=device)
+    if os.path.exists(done) and os.path.exists(prefix + ".bwt"):
+        return ix.read_bwt(prefix + ".bwt")
+    if str(device).startswith("cuda"):
+        m, words, _, _ = nv.build_bwt(ix.encode_forward(ref), 0)
+        L2 = [int(m.L2[i]) for i in range(5)]
+        ix.write_bwt(prefix + ".bwt", int(m.primary), L2, words)
         open(done, "w").close()
+        return {"primary": int(m.primary), "L2": L2, "seq_len": L2[4], "bwt_size": int(words.size)}, words
+    ix.build_index(ref, prefix, device=device)
+    open(done, "w").close()
     return ix.read_bwt(prefix + ".bwt")
 
 
@@ -104,4 +114,54 @@ def adna_mixed(workdir: str, ref_bp: int = 100_000_000, n_reads: int = 10_000_00
     reads, lens = synth.adna_reads(seeds[1], ref, n_reads, workers=workers)
     w = Workload("adna_mixed", prefix, meta, words, reads, lens, list(ADNA_OPTS))
     w.timings.update(index_s=t_idx, reads_s=time.time() - t - t_idx)
+    return w.prepare()
+
+
+def human_scale(workdir: str, ref_bp: int = 3_100_000_000, n_reads: int = 50_000_000,
+                seeds: tuple[int, int] = (301, 302), device: str = "cuda", workers: int = 8,
+                lo: int = 30, hi: int = 150) -> Workload:
+    """Human-scale reference (24 contigs, 41% GC, 45% interspersed repeats from 50 families at
+    2-20% divergence, tandem repeats of period 2-200); reads of uniform length lo..hi with
+    deamination and substitutions; aln -l 1024 -n 0.01 -o 2."""
+    os.makedirs(workdir, exist_ok=True)
+    t = time.time()
+    ref = np.concatenate([c for _, c in synth.human_like_reference(seeds[0], ref_bp)])
+    t_ref = time.time() - t
+    prefix = os.path.join(workdir, f"human_{ref_bp}_{seeds[0]}")
+    meta, words = _index(ref, prefix, device)
+    t_idx = time.time() - t - t_ref
+    reads, lens = synth.uniform_length_reads(seeds[1], ref, n_reads, lo, hi, workers=workers)
+    w = Workload("human_scale", prefix, meta, words, reads, lens, list(ADNA_OPTS))
+    w.timings.update(ref_s=t_ref, index_s=t_idx, reads_s=time.time() - t - t_ref - t_idx)
+    w.ref = ref
+    return w.prepare()
+
+
+DEEP_OPTS = ["-n", "0.04", "-o", "2", "-e", "6", "-k", "2"]
+
+
+def deep_copies(workdir: str, base_bp: int = 2_500_000_000, copies: int = 8, n_pairs: int = 20_000_000,
+                seeds: tuple[int, int] = (501, 502), device: str = "cuda", workers: int = 8, progress=None) -> Workload:
+    """Deep repeats: a random base genome and `copies` copies with 1-3% substitutions and 0.2%
+    indels (20 Gbp at the defaults, n = 4e10 > 2^32); 2 x 150 bp pairs (insert N(350, 50), 2%
+    errors), aln -n 0.04 -o 2 -e 6 -k 2. The two mates' files are one read set here, mate 1 then
+    mate 2: every read is 150 bp, so each batch's options equal those of two separate runs."""
+    os.makedirs(workdir, exist_ok=True)
+    t = time.time()
+    def note(i):
+        if progress:
+            progress(f"copy {i + 1} of {copies} made ({time.time() - t:.0f} s)")
+
+    ref = np.concatenate([c for _, c in synth.diverged_copies_reference(seeds[0], base_bp, copies, note)])
+    t_ref = time.time() - t
+    prefix = os.path.join(workdir, f"deep_{base_bp}x{copies}_{seeds[0]}")
+    meta, words = _index(ref, prefix, device)
+    t_idx = time.time() - t - t_ref
+    m1, l1, m2, l2 = synth.paired_reads_chunked(seeds[1], ref, n_pairs, workers=workers)
+    reads = np.concatenate([m1, m2])
+    del m1, m2
+    lens = np.concatenate([l1, l2])
+    w = Workload("deep_copies", prefix, meta, words, reads, lens, list(DEEP_OPTS))
+    w.timings.update(ref_s=t_ref, index_s=t_idx, reads_s=time.time() - t - t_ref - t_idx)
+    w.ref = ref
     return w.prepare()
