@@ -1189,6 +1189,57 @@ __global__ void __launch_bounds__(128) k_search(KArgs a)
 }
 
 // ---------------------------------------------------------------------------
+// SA value of BWT rows (bwt_sa with bwt_invPsi, upstream bwt.c:53-59, :88-96): LF steps from each
+// row to the nearest row divisible by the sample interval (fewer than intv steps, one 64-byte block
+// each), then that row's sample plus the steps. Row 0, the $ row, counts as -1 as bwa sets it (a
+// walk that ends there gives steps - 1). samples[j] is the suffix at row (j + 1) * intv; rows are
+// at most seq_len (checked by the caller).
+__global__ void __launch_bounds__(256) k_sa_walk(const uint32_t *__restrict__ bwt, DMeta m,
+                                                  const uint64_t *__restrict__ samples, uint64_t intv,
+                                                  const uint64_t *__restrict__ rows, uint64_t n, uint64_t *__restrict__ pos,
+                                                  const int32_t *__restrict__ ref_len, int32_t *__restrict__ strand)
+{
+    const L2Sel L2(m);
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
+        uint64_t k = rows[i], sa = 0;
+        while (k & (intv - 1)) {
+            ++sa;
+            if (k == m.primary) {
+                k = 0;
+                break;
+            }
+            const uint64_t x = k - (k > m.primary);
+            uint4 B[4];
+            load_block(bwt, x >> 7, B);
+            const int r = (int)(x & 127);
+            const uint32_t hw = r < 32 ? B[2].x : r < 64 ? B[2].y : r < 96 ? B[2].z : B[2].w;
+            const uint32_t lw = r < 32 ? B[3].x : r < 64 ? B[3].y : r < 96 ? B[3].z : B[3].w;
+            const int sh = 31 - (r & 31);
+            const int c = (int)(((hw >> sh) & 1u) << 1 | ((lw >> sh) & 1u));
+            k = L2[c] + block_occ1(B, r, c);
+        }
+        uint64_t p = sa + (k == 0 ? ~0ull : samples[k / intv - 1]);
+        if (ref_len) {
+            // bwa_sa2pos (upstream bwase.c:112-123): a hit across the forward/reverse boundary is -1;
+            // on the reverse half, the forward position of the hit's first base, strand 0
+            const uint64_t l_pac = m.seq_len >> 1;
+            const uint64_t rl = (uint64_t)(int64_t)ref_len[i];
+            int st = 0;
+            if (p < l_pac && l_pac < p + rl) {
+                p = ~0ull;
+            } else {
+                const bool rev = p >= l_pac;
+                if (rev) p = (l_pac << 1) - 1 - p;
+                st = rev ? 0 : 1;
+                if (rev) p = p + 1 < rl ? 0 : p - rl + 1;
+            }
+            strand[i] = st;
+        }
+        pos[i] = p;
+    }
+}
+
+// ---------------------------------------------------------------------------
 // Wave-per-read engine for heavy reads.
 //
 // Between two hits the search's state (max_diff, widths, best score) is fixed,
@@ -2200,6 +2251,10 @@ struct galn_hip_index {
     uint32_t *d_pool_link = nullptr, *d_gnext = nullptr, *d_gbump = nullptr;
     unsigned long long *d_ghead = nullptr;
     uint32_t n_gran = 0;
+    // SA samples (galn_hip_index_set_sa): the suffix at rows intv, 2 intv, ...
+    uint64_t *d_sa = nullptr;
+    uint64_t n_sa = 0;
+    uint32_t sa_intv = 0;
     std::atomic<int> n_readsets{0};  // read sets not yet freed (they use the stream and pool)
     std::atomic<int> n_servers{0};   // open servers: the index then belongs to their workers
 };
@@ -3172,7 +3227,7 @@ int galn_hip_index_close(galn_hip_index *idx)
     (void)hipSetDevice(idx->device);
     if (idx->stream) (void)hipStreamSynchronize(idx->stream);
     (void)hipFree(idx->d_bwt);
-    void *pool[] = {idx->d_pool_chk, idx->d_pool_link, idx->d_gnext, idx->d_gbump, idx->d_ghead};
+    void *pool[] = {idx->d_pool_chk, idx->d_pool_link, idx->d_gnext, idx->d_gbump, idx->d_ghead, idx->d_sa};
     for (void *p : pool)
         if (p) (void)hipFree(p);
     if (idx->stream) (void)hipStreamDestroy(idx->stream);
@@ -3425,6 +3480,72 @@ int galn_hip_search_batched(galn_hip_index *idx, const galn_hip_reads *reads, ga
                             const galn_hip_tune_t *tune)
 {
     return search_once(idx, reads, out, 1, flags, tune);
+}
+
+}  // extern "C"
+
+extern "C" {
+
+int galn_hip_index_set_sa(galn_hip_index *idx, const uint64_t *samples, uint64_t n_samples, uint32_t intv)
+{
+    if (!idx || (!samples && n_samples)) return fail(GALN_HIP_E_ARG, "index_set_sa: null argument");
+    int r = busy_check(idx, "index_set_sa");
+    if (r) return r;
+    if (intv == 0 || (intv & (intv - 1)) != 0) return fail(GALN_HIP_E_ARG, "index_set_sa: interval %u is not a power of 2", intv);
+    const uint64_t want = (idx->hmeta.seq_len + intv) / intv - 1;
+    if (n_samples != want)
+        return fail(GALN_HIP_E_ARG, "index_set_sa: %llu samples, %llu expected for interval %u",
+                    (unsigned long long)n_samples, (unsigned long long)want, intv);
+    HIP_OK(hipSetDevice(idx->device));
+    if (idx->d_sa) HIP_OK(hipFree(idx->d_sa));
+    idx->d_sa = nullptr;
+    idx->n_sa = 0;
+    HIP_OK(hipMalloc(&idx->d_sa, std::max<uint64_t>(n_samples, 1) * 8));
+    if (n_samples) HIP_OK(hipMemcpy(idx->d_sa, samples, n_samples * 8, hipMemcpyHostToDevice));
+    idx->n_sa = n_samples;
+    idx->sa_intv = intv;
+    return 0;
+}
+
+int galn_hip_sa_positions(galn_hip_index *idx, const uint64_t *rows, uint64_t n, uint64_t *pos, const int32_t *ref_len,
+                          int32_t *strand)
+{
+    if (!idx || (n && (!rows || !pos)) || (!ref_len != !strand))
+        return fail(GALN_HIP_E_ARG, "sa_positions: null argument (ref_len and strand come together)");
+    int r = busy_check(idx, "sa_positions");
+    if (r) return r;
+    if (!idx->sa_intv) return fail(GALN_HIP_E_ARG, "sa_positions: no SA samples (galn_hip_index_set_sa)");
+    for (uint64_t i = 0; i < n; ++i)
+        if (rows[i] > idx->hmeta.seq_len)
+            return fail(GALN_HIP_E_ARG, "sa_positions: row %llu beyond %llu", (unsigned long long)rows[i],
+                        (unsigned long long)idx->hmeta.seq_len);
+    if (n == 0) return 0;
+    HIP_OK(hipSetDevice(idx->device));
+    const uint64_t chunk = std::min<uint64_t>(n, 1ull << 24);
+    uint64_t *d = nullptr;
+    HIP_OK(hipMalloc(&d, chunk * (ref_len ? 24 : 16)));
+    int32_t *const d_rl = ref_len ? reinterpret_cast<int32_t *>(d + 2 * chunk) : nullptr;
+    int32_t *const d_st = ref_len ? d_rl + chunk : nullptr;
+    hipStream_t s = idx->stream;
+    int rc = 0;
+    for (uint64_t a = 0; a < n && !rc; a += chunk) {
+        const uint64_t m = std::min(chunk, n - a);
+        if (hipMemcpyAsync(d, rows + a, m * 8, hipMemcpyHostToDevice, s) != hipSuccess ||
+            (ref_len && hipMemcpyAsync(d_rl, ref_len + a, m * 4, hipMemcpyHostToDevice, s) != hipSuccess)) {
+            rc = fail(GALN_HIP_E_DEVICE, "sa_positions: copy failed");
+            break;
+        }
+        hipLaunchKernelGGL(k_sa_walk, dim3((unsigned)std::min<uint64_t>((m + 255) / 256, 65535)), dim3(256), 0, s,
+                           (const uint32_t *)idx->d_bwt, idx->meta, (const uint64_t *)idx->d_sa, (uint64_t)idx->sa_intv,
+                           (const uint64_t *)d, m, d + chunk, (const int32_t *)d_rl, d_st);
+        if (hipGetLastError() != hipSuccess ||
+            hipMemcpyAsync(pos + a, d + chunk, m * 8, hipMemcpyDeviceToHost, s) != hipSuccess ||
+            (ref_len && hipMemcpyAsync(strand + a, d_st, m * 4, hipMemcpyDeviceToHost, s) != hipSuccess) ||
+            hipStreamSynchronize(s) != hipSuccess)
+            rc = fail(GALN_HIP_E_DEVICE, "sa_positions: kernel or copy failed");
+    }
+    (void)hipFree(d);
+    return rc;
 }
 
 }  // extern "C"

@@ -31,7 +31,8 @@ HIP_SYMBOLS = (
     "galn_hip_readset_reload", "galn_hip_readset_autotune", "galn_hip_readset_kernel_ms", "galn_hip_readset_launches",
     "galn_hip_readset_engine_stats", "galn_hip_server_open", "galn_hip_server_submit", "galn_hip_server_wait",
     "galn_hip_server_result", "galn_hip_server_stats", "galn_hip_server_close", "galn_hip_gather_ceiling",
-    "galn_hip_search_device_noalloc", "galn_hip_bwt_words", "galn_hip_build_bwt",
+    "galn_hip_search_device_noalloc", "galn_hip_bwt_words", "galn_hip_build_bwt", "galn_hip_index_set_sa",
+    "galn_hip_sa_positions",
 )
 HOST_SYMBOLS = (
     "galn_gap_opt_default", "galn_parse_aln_args", "galn_cal_maxdiff", "galn_batch_opt", "galn_read_maxdiff",
@@ -135,6 +136,8 @@ def hip_lib():
         L.galn_hip_bwt_words.restype = C.c_uint64
         L.galn_hip_bwt_words.argtypes = [C.c_uint64]
         L.galn_hip_build_bwt.argtypes = [P, C.c_uint64, C.c_int, C.POINTER(BwtMeta), P, P, P]
+        L.galn_hip_index_set_sa.argtypes = [P, P, C.c_uint64, C.c_uint32]
+        L.galn_hip_sa_positions.argtypes = [P, P, C.c_uint64, P, P, P]
         L.galn_hip_get_limits.argtypes = [C.POINTER(Limits)]
         L.galn_hip_gather_ceiling.argtypes = [C.c_int, C.c_uint64, C.c_int, C.c_int, C.c_int,
                                               C.POINTER(C.c_double), C.POINTER(C.c_double)]
@@ -325,6 +328,28 @@ class Index:
             raise GalnError(f"index_open failed ({rc}): {last_error()}")
         self.h = h
         self._words = None
+
+    def set_sa(self, samples: np.ndarray, intv: int = 32) -> None:
+        """galn_hip_index_set_sa: the SA samples of rows intv, 2 intv, ... (a .sa file's body)."""
+        sm = np.ascontiguousarray(samples, dtype=np.uint64)
+        rc = hip_lib().galn_hip_index_set_sa(self.h, _p(sm), int(sm.size), int(intv))
+        if rc != 0:
+            raise GalnError(f"index_set_sa failed ({rc}): {last_error()}")
+
+    def sa_positions(self, rows, ref_len=None):
+        """galn_hip_sa_positions: bwt_sa of each row, or with ref_len (per row) bwa_sa2pos's
+        (position, strand)."""
+        r = np.ascontiguousarray(rows, dtype=np.uint64)
+        pos = np.empty(max(r.size, 1), np.uint64)
+        if ref_len is None:
+            rc = hip_lib().galn_hip_sa_positions(self.h, _p(r), int(r.size), _p(pos), None, None)
+        else:
+            rl = np.ascontiguousarray(np.broadcast_to(ref_len, r.shape), dtype=np.int32)
+            st = np.empty(max(r.size, 1), np.int32)
+            rc = hip_lib().galn_hip_sa_positions(self.h, _p(r), int(r.size), _p(pos), _p(rl), _p(st))
+        if rc != 0:
+            raise GalnError(f"sa_positions failed ({rc}): {last_error()}")
+        return pos[: r.size] if ref_len is None else (pos[: r.size], st[: r.size])
 
     def close(self):
         if self.h:

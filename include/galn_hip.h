@@ -238,6 +238,22 @@ int galn_hip_server_stats(galn_hip_server *srv, double out[8]);
 /* Finishes the requests already submitted, then stops the worker. */
 int galn_hip_server_close(galn_hip_server *srv);
 
+/* SA samples for the position step: samples[j] is the text position of the suffix at BWT row
+ * (j + 1) * intv, j < (seq_len + intv) / intv - 1 -- the body of a .sa file after its 56-byte
+ * header (bwt_dump_sa/bwt_restore_sa, src/index/idxcommon.c:137-153), which galn_hip_build_bwt
+ * also returns. Uploaded once; kept with the index. */
+int galn_hip_index_set_sa(galn_hip_index *idx, const uint64_t *samples, uint64_t n_samples, uint32_t intv);
+
+/* Text positions of BWT rows on the GPU (replaces bwt_sa / bwa_sa2pos of samse, upstream
+ * src/samse/bwa_upstream/bwt.c:53-96 and bwase.c:112-123, and the LF-walk kernel of
+ * src/index/gpu_index.cu:93-130): from each row (<= seq_len, e.g. a .sai record's k) LF steps to the
+ * nearest sampled row. pos[i] = bwt_sa(rows[i]) on the forward-reverse text (row 0 gives all ones,
+ * bwa's -1). With ref_len (and strand) given: bwa_sa2pos instead -- the forward-strand position of
+ * the hit's first base for a hit of ref_len[i] reference bases, strand[i] = 1 forward / 0 reverse,
+ * and all ones for a hit across the forward/reverse boundary. Host arrays. */
+int galn_hip_sa_positions(galn_hip_index *idx, const uint64_t *rows, uint64_t n, uint64_t *pos, const int32_t *ref_len,
+                          int32_t *strand);
+
 /* u32 words of a .bwt body for a BWT of n symbols: 8 * (ceil(n/128) + 1) + ceil(n/16)
  * (galn_bwt_words, src/index/idxcommon.h:40-44). */
 uint64_t galn_hip_bwt_words(uint64_t n);

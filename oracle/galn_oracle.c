@@ -108,6 +108,54 @@ static inline uint64_t query_bytes(const orc_meta_t *m, uint64_t a, uint64_t b)
     return ba == bb ? 64 : 128;
 }
 
+/* the symbol at position x of the $-less BWT (upstream bwt.h bwt_B0) */
+static inline int bwt_sym(const uint32_t *bwt, uint64_t x)
+{
+    const uint32_t w = bwt[((x >> 7) << 4) + 8 + ((x & 127) >> 4)];
+    return (int)((w >> (30 - 2 * (x & 15))) & 3u);
+}
+
+/* bwt_invPsi: the row one text position to the left (upstream bwt.c:53-59) */
+uint64_t orc_inv_psi(const uint32_t *bwt, const orc_meta_t *m, uint64_t k)
+{
+    uint64_t x;
+    int c;
+    if (k == m->primary) return 0;
+    x = k - (k > m->primary);
+    c = bwt_sym(bwt, x);
+    return m->L2[c] + orc_occ(bwt, m, k, c);
+}
+
+/* bwt_sa: the text position of row k (upstream bwt.c:88-96), with the samples of rows intv, 2 intv,
+ * ... in samples[0], samples[1], ... and row 0 (the $ row) counted as -1, as bwt_restore_sa sets it */
+uint64_t orc_sa(const uint32_t *bwt, const orc_meta_t *m, const uint64_t *samples, uint64_t intv, uint64_t k)
+{
+    uint64_t sa = 0, mask = intv - 1;
+    while (k & mask) {
+        ++sa;
+        k = orc_inv_psi(bwt, m, k);
+    }
+    return sa + (k == 0 ? NEG1 : samples[k / intv - 1]);
+}
+
+/* bwa_sa2pos (upstream bwase.c:112-123, bntseq.h bns_depos): the forward-strand position of the
+ * first base of a hit of ref_len bases at row k; *strand = 1 forward, 0 reverse; -1 across the
+ * forward/reverse boundary */
+uint64_t orc_sa2pos(const uint32_t *bwt, const orc_meta_t *m, const uint64_t *samples, uint64_t intv, uint64_t k,
+                    int ref_len, int *strand)
+{
+    const uint64_t l_pac = m->seq_len >> 1;
+    uint64_t pos = orc_sa(bwt, m, samples, intv, k);
+    int is_rev;
+    *strand = 0;
+    if (pos < l_pac && l_pac < pos + (uint64_t)(int64_t)ref_len) return NEG1;
+    is_rev = pos >= l_pac;
+    if (is_rev) pos = (l_pac << 1) - 1 - pos;
+    *strand = !is_rev;
+    if (is_rev) pos = pos + 1 < (uint64_t)(int64_t)ref_len ? 0 : pos - (uint64_t)(int64_t)ref_len + 1;
+    return pos;
+}
+
 /* bwt_2occ for one base: src/bwtsearch.h:73-108 */
 static inline void occ2(const uint32_t *bwt, const orc_meta_t *m, uint64_t k, uint64_t l, int c,
                         uint64_t *ok, uint64_t *ol)

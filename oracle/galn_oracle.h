@@ -68,6 +68,13 @@ uint64_t orc_match_exact(const uint32_t *bwt, const orc_meta_t *m, int len, cons
 int      orc_cal_width(const uint32_t *bwt, const orc_meta_t *m, int len, const uint8_t *str,
                        int comp, orc_width_t *width);
 
+/* bwt_invPsi and bwt_sa (upstream bwt.c:53-59, :88-96): LF walk to a sampled row */
+uint64_t orc_inv_psi(const uint32_t *bwt, const orc_meta_t *m, uint64_t k);
+uint64_t orc_sa(const uint32_t *bwt, const orc_meta_t *m, const uint64_t *samples, uint64_t intv, uint64_t k);
+/* bwa_sa2pos (upstream bwase.c:112-123) */
+uint64_t orc_sa2pos(const uint32_t *bwt, const orc_meta_t *m, const uint64_t *samples, uint64_t intv, uint64_t k,
+                    int ref_len, int *strand);
+
 /* ---- options ---- */
 int orc_cal_maxdiff(int l, double err, double thres);
 /* per-batch options from the .sai header fields; returns n_stacks */

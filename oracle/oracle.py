@@ -68,6 +68,12 @@ def lib():
         L.orc_result_nrec.argtypes = [P]
         L.orc_result_copy.argtypes = [P] * 9  # handle + 8 outputs
         L.orc_result_free.argtypes = [P]
+        L.orc_inv_psi.restype = C.c_uint64
+        L.orc_inv_psi.argtypes = [P, C.POINTER(Meta), C.c_uint64]
+        L.orc_sa.restype = C.c_uint64
+        L.orc_sa.argtypes = [P, C.POINTER(Meta), P, C.c_uint64, C.c_uint64]
+        L.orc_sa2pos.restype = C.c_uint64
+        L.orc_sa2pos.argtypes = [P, C.POINTER(Meta), P, C.c_uint64, C.c_uint64, C.c_int, C.POINTER(C.c_int)]
         _lib = L
     return _lib
 
@@ -84,6 +90,32 @@ def make_meta(meta: dict) -> Meta:
 
 def _ptr(a: np.ndarray):
     return a.ctypes.data_as(C.c_void_p)
+
+
+def sa_positions(words: np.ndarray, meta: dict, samples: np.ndarray, rows, intv: int = 32) -> np.ndarray:
+    """bwt_sa for each row (upstream bwt.c:88-96): text position of the suffix at that row, from the
+    SA samples of rows intv, 2 intv, ... (row 0, the $ row, gives all ones as bwa's -1)."""
+    L = lib()
+    m = make_meta(meta)
+    w = np.ascontiguousarray(words, dtype=np.uint32)
+    sm = np.ascontiguousarray(samples, dtype=np.uint64)
+    return np.array([L.orc_sa(_ptr(w), C.byref(m), _ptr(sm), intv, int(k)) for k in rows], dtype=np.uint64)
+
+
+def sa2pos(words: np.ndarray, meta: dict, samples: np.ndarray, rows, ref_len, intv: int = 32):
+    """bwa_sa2pos for each row (upstream bwase.c:112-123): (forward-strand position of the hit's
+    first base, strand 1 forward / 0 reverse); all ones across the forward/reverse boundary."""
+    L = lib()
+    m = make_meta(meta)
+    w = np.ascontiguousarray(words, dtype=np.uint32)
+    sm = np.ascontiguousarray(samples, dtype=np.uint64)
+    rl = np.broadcast_to(np.asarray(ref_len), np.shape(rows))
+    pos, st = [], []
+    s = C.c_int()
+    for k, r in zip(rows, rl):
+        pos.append(L.orc_sa2pos(_ptr(w), C.byref(m), _ptr(sm), intv, int(k), int(r), C.byref(s)))
+        st.append(s.value)
+    return np.array(pos, dtype=np.uint64), np.array(st, dtype=np.int32)
 
 
 # --------------------------------------------------------------------------

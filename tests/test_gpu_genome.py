@@ -102,13 +102,24 @@ def test_deep_repeats_above_2_32_symbols(tmp_path):
     ix.write_sa(prefix + ".sa", int(meta.primary), L2, int(meta.seq_len), sa)
     assert fx.md5_file(prefix + ".bwt") == exp["bwt_md5"]
     assert fx.md5_file(prefix + ".sa") == exp["sa_md5"]
-    del sa
     os.remove(prefix + ".sa")
     m1, l1, _, _ = synth.paired_reads(93, ref, 20_000)
     res = _search_and_check(meta, words, m1, l1, fx.DEEP)
     nr = int(np.maximum(res.n_aln, 0).sum())
     assert int(res.rec[:nr, 2].max()) > 2**32  # intervals past 2^32 reached the .sai records
     print("pops histogram:", _pops_histogram(res.pops), flush=True)
+    # positions of records' rows on the GPU equal the oracle's (positions past 2^32 included)
+    rows = np.random.default_rng(4).choice(res.rec[:nr, 1], size=3000, replace=False).astype(np.uint64)
+    with nv.Index(words, meta, 0) as index:
+        index.set_sa(sa)
+        pos, st = index.sa_positions(rows, 150)
+        pf = index.sa_positions(rows)
+    m = {"primary": int(meta.primary), "L2": L2, "seq_len": int(meta.seq_len), "bwt_size": int(meta.bwt_size)}
+    np.testing.assert_array_equal(pf, orc.sa_positions(words, m, sa, rows))
+    ep, es = orc.sa2pos(words, m, sa, rows, 150)
+    np.testing.assert_array_equal(pos, ep)
+    np.testing.assert_array_equal(st, es)
+    assert int(pf.max()) > 2**32
 
 
 def test_human_scale_index_and_search(tmp_path):

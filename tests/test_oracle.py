@@ -255,3 +255,57 @@ def test_max_diff_zero_is_exact_match():
             rec = res.records(i)[0]
             assert (int(rec[1]), int(rec[2])) == iv
             assert int(rec[0]) == 0  # no mismatch, no gap, score 0
+
+
+def _sa_index(f2):
+    """(meta, words, full suffix array, samples of rows 32, 64, ...) of T = F . revcomp(F)."""
+    import torch
+    f = torch.as_tensor(f2, dtype=torch.uint8)
+    text = torch.cat([f, (3 - f).flip(0)])
+    n = int(text.numel())
+    sa = ix.suffix_array(text).numpy().astype(np.uint64)
+    primary, L = ix.bwt_from_forward(f2)
+    L2, words = ix.pack_bwt(L)
+    meta = {"primary": primary, "L2": L2, "seq_len": n, "bwt_size": int(words.size)}
+    return meta, words, sa, sa[32 * np.arange(1, (n + 32) // 32) - 1]
+
+
+@pytest.mark.parametrize("kind", ["random", "repeats"])
+def test_oracle_sa_equals_sorted_suffixes(kind):
+    """bwt_sa of every row (upstream bwt.c:88-96) is the suffix sorted to that row; row 0 is bwa's -1."""
+    rng = np.random.default_rng(11)
+    f2 = rng.integers(0, 4, 4001).astype(np.uint8)
+    if kind == "repeats":
+        f2[2000:3000] = f2[100:1100]
+        f2[3500:3800] = np.tile(np.array([0, 1], np.uint8), 150)
+    meta, words, sa, samples = _sa_index(f2)
+    pos = orc.sa_positions(words, meta, samples, np.arange(meta["seq_len"] + 1))
+    np.testing.assert_array_equal(pos, np.concatenate([[np.uint64(2**64 - 1)], sa]))
+
+
+def test_oracle_sa2pos_exact_hits_land_on_the_read(golden_dir):
+    """bwa_sa2pos (upstream bwase.c:112-123): for every exact hit (no mismatch or gap), the forward
+    reference at the position is the read itself on strand 0 and its reverse complement on strand 1."""
+    w = golden_inputs("plain_small", golden_dir)
+    contigs = fx.reference_contigs(fx.SPECS["plain_small"])
+    ref = np.concatenate([c for _, c in contigs])
+    f2 = ix.encode_forward(ref)
+    meta, words, _, samples = _sa_index(f2)
+    r = load_reads(w["reads"][0])
+    o = orc.parse_aln_opts([])
+    res = orc.run_workload(words, meta, r["query"], r["qoff"], r["qlen"], o)
+    comp = np.array([3, 2, 1, 0, 4], np.uint8)
+    checked = 0
+    for i in range(300):
+        L = int(r["qlen"][i])
+        q = r["query"][int(r["qoff"][i]):int(r["qoff"][i]) + L]
+        read = comp[q[::-1]]  # the query is the read's reverse complement
+        for rec in res.records(i):
+            if int(rec[0]) & 0xFFFFFF:  # mismatches or gaps
+                continue
+            for k in range(int(rec[1]), min(int(rec[2]), int(rec[1]) + 3) + 1):
+                (p,), (s,) = orc.sa2pos(words, meta, samples, [k], L)
+                seg = f2[int(p):int(p) + L]
+                assert np.array_equal(seg, read if s == 0 else comp[read[::-1]]), (i, k, int(p), int(s))
+                checked += 1
+    assert checked > 100
