@@ -167,7 +167,7 @@ def measured_traffic(kernel: str, args, tuned: dict) -> int | None:
     return value
 
 
-def serving_sweep(w, index, nv, loads, seconds: float, max_req: int, seed: int = 7) -> dict:
+def serving_sweep(w, index, nv, loads, seconds: float, max_req: int, seed: int = 7, slots: int = 0) -> dict:
     """Serving loop under Poisson arrivals. Requests are slices of the workload's reads of
     log-uniform size (1000 .. max_req reads), each searched with BWA's default options or the
     aDNA options (50/50), batch options computed per request as `galn aln` would. The
@@ -232,10 +232,11 @@ def serving_sweep(w, index, nv, loads, seconds: float, max_req: int, seed: int =
 
     out = {"request_reads": f"log-uniform 1000..{max_req}", "options": "50/50 default / aln -l 1024 -n 0.01 -o 2",
            "loads": []}
-    with nv.Server(index) as srv:
+    with nv.Server(index, slots=slots) as srv:
         warm = [make_request() for _ in range(2)]
         run(srv, warm, None)
-        peak_reqs = [make_request() for _ in range(12)]
+        # the sustained rate over enough requests to keep every launch slot busy
+        peak_reqs = [make_request() for _ in range(48)]
         lats, el = run(srv, peak_reqs, None)
         n_reads = sum(int(r[2].size) for r in peak_reqs)
         peak = n_reads / el
@@ -254,6 +255,34 @@ def serving_sweep(w, index, nv, loads, seconds: float, max_req: int, seed: int =
             log(f"serving: load {load:.2f}: {n_req} requests, p50 {rec['p50_ms']:.1f} ms, p99 {rec['p99_ms']:.1f} ms, "
                 f"{rec['reads_per_s']:.3e} reads/s")
         out["server_stats"] = srv.stats()
+    # the batched form on exactly the peak phase's requests (one read set, each request's reads
+    # with its own batch options): the throughput the serving peak is to be compared with
+    parts = list(zip(*[r for r in peak_reqs]))
+    qs, qos, qls, mds, bids, bopt_l, nst_l = parts
+    qoff_all, q_all, batch_all, base, b0 = [], [], [], 0, 0
+    bopts_all = (nv.SearchOpt * sum(len(b) for b in bopt_l))()
+    nst_all = np.concatenate(nst_l).astype(np.int32)
+    k = 0
+    for q, qo, bid, bo in zip(qs, qos, bids, bopt_l):
+        q_all.append(q)
+        qoff_all.append(qo + np.uint64(base))
+        batch_all.append(bid + b0)
+        base += int(q.size)
+        for x in bo:
+            bopts_all[k] = x
+            k += 1
+        b0 += len(bo)
+    rs_mix = nv.ReadSet(index, np.concatenate(q_all), np.concatenate(qoff_all), np.concatenate(qls),
+                        np.concatenate(mds), np.concatenate(batch_all).astype(np.int32), bopts_all, nst_all)
+    rs_mix.search(nv.FORM_BATCHED)
+    t0 = time.perf_counter()
+    rs_mix.search(nv.FORM_BATCHED)
+    rs_mix.results()
+    el = time.perf_counter() - t0
+    rs_mix.close()
+    out["batched_same_requests_reads_per_s"] = n_reads / el
+    out["peak_over_batched_same_requests"] = peak / (n_reads / el)
+    log(f"serving: the batched form on the same {len(peak_reqs)} requests: {n_reads / el:.3e} reads/s")
     return out
 
 
@@ -275,6 +304,7 @@ def main() -> int:
                          "slower; -1: 1M, or 200k for the genome-scale workloads)")
     ap.add_argument("--serve-seconds", type=float, default=8.0, help="arrival time per offered load (0: no serving)")
     ap.add_argument("--serve-max-request", type=int, default=1_000_000)
+    ap.add_argument("--serve-slots", type=int, default=0, help="serving launches on the device at once (0: library default)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="also copy a buffer of this many MiB device to device three times (the known-size "
                          "kernel that scales the FETCH_SIZE / WRITE_SIZE counters)")
@@ -488,7 +518,8 @@ def main() -> int:
     if args.serve_seconds > 0 and args.workload in ("adna", "human"):
         pool = serve_pool if serve_pool is not None else w
         out["serving"] = serving_sweep(pool, index, nv, (0.25, 0.5, 0.75, 0.9), args.serve_seconds,
-                                       min(args.serve_max_request, pool.n))
+                                       min(args.serve_max_request, pool.n), slots=args.serve_slots)
+        out["serving"]["peak_over_batched"] = out["serving"]["peak_reads_per_s"] / value
     rs.close()
     index.close()
     if not args.no_cpu:

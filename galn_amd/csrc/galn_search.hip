@@ -2239,18 +2239,33 @@ struct Tier {
 
 }  // namespace
 
-struct galn_hip_index {
-    int device = 0;
+// Where a search runs: a stream and a stack pool (grown on demand). The read sets of an index
+// share its context; a read set of a server slot has one of its own, so that slots' launches run
+// at the same time.
+struct SearchCtx {
     hipStream_t stream = nullptr;
-    uint32_t *d_bwt = nullptr;
-    DMeta meta{};
-    galn_bwt_meta_t hmeta{};
-    int n_cu = 0;
-    // the stack pool (grown on demand, shared by the searches on this index)
     Ent *d_pool_chk = nullptr;
     uint32_t *d_pool_link = nullptr, *d_gnext = nullptr, *d_gbump = nullptr;
     unsigned long long *d_ghead = nullptr;
     uint32_t n_gran = 0;
+    // frees the pool (not the stream)
+    void free_pool()
+    {
+        void *p[] = {d_pool_chk, d_pool_link, d_gnext, d_gbump, d_ghead};
+        for (void *x : p)
+            if (x) (void)hipFree(x);
+        d_pool_chk = nullptr, d_pool_link = nullptr, d_gnext = nullptr, d_gbump = nullptr, d_ghead = nullptr;
+        n_gran = 0;
+    }
+};
+
+struct galn_hip_index {
+    int device = 0;
+    uint32_t *d_bwt = nullptr;
+    DMeta meta{};
+    galn_bwt_meta_t hmeta{};
+    int n_cu = 0;
+    SearchCtx ctx0;  // the stream and stack pool of the index's own read sets
     // SA samples (galn_hip_index_set_sa): the suffix at rows intv, 2 intv, ...
     uint64_t *d_sa = nullptr;
     uint64_t n_sa = 0;
@@ -2276,6 +2291,8 @@ struct GraphKey {
 
 struct galn_hip_readset {
     galn_hip_index *idx = nullptr;
+    SearchCtx *ctx = nullptr;   // &idx->ctx0, or own_ctx
+    SearchCtx *own_ctx = nullptr;
     int n = 0;
     int n_batches = 0;
     int seed_len = 0;
@@ -2392,7 +2409,7 @@ uint64_t pool_test_cap()
 }
 
 // The stack pool: `want` granules of 64 chunks (16 KB each), within half the free memory.
-int ensure_pool(galn_hip_index *idx, uint64_t want)
+int ensure_pool(SearchCtx *idx, uint64_t want)
 {
     const uint64_t gbytes = (uint64_t)GRAN * CH * sizeof(Ent);
     size_t freeb = 0, totalb = 0;
@@ -2443,20 +2460,20 @@ int ensure_pool(galn_hip_index *idx, uint64_t want)
 }
 
 template <bool TRACE, bool PERSIST>
-int launch_one(galn_hip_index *idx, const KArgs &a, int block, int grid, int buckets)
+int launch_one(hipStream_t st, const KArgs &a, int block, int grid, int buckets)
 {
     const size_t lds = (size_t)block * buckets * sizeof(uint32_t);
-    hipLaunchKernelGGL((k_search<TRACE, PERSIST>), dim3(grid), dim3(block), lds, idx->stream, a);
+    hipLaunchKernelGGL((k_search<TRACE, PERSIST>), dim3(grid), dim3(block), lds, st, a);
     HIP_OK(hipGetLastError());
     return 0;
 }
 
-int launch(galn_hip_index *idx, const KArgs &a, int block, int grid, bool persist, bool trace, int buckets)
+int launch(hipStream_t st, const KArgs &a, int block, int grid, bool persist, bool trace, int buckets)
 {
-    if (trace) return persist ? launch_one<true, true>(idx, a, block, grid, buckets)
-                              : launch_one<true, false>(idx, a, block, grid, buckets);
-    return persist ? launch_one<false, true>(idx, a, block, grid, buckets)
-                   : launch_one<false, false>(idx, a, block, grid, buckets);
+    if (trace) return persist ? launch_one<true, true>(st, a, block, grid, buckets)
+                              : launch_one<true, false>(st, a, block, grid, buckets);
+    return persist ? launch_one<false, true>(st, a, block, grid, buckets)
+                   : launch_one<false, false>(st, a, block, grid, buckets);
 }
 
 int fill_u32(hipStream_t st, void *p, uint32_t v, size_t bytes)
@@ -2469,7 +2486,7 @@ int fill_u32(hipStream_t st, void *p, uint32_t v, size_t bytes)
 }
 #define FILL(p, v, bytes)                                                                              \
     do {                                                                                               \
-        const int fr_ = fill_u32(idx->stream, (p), (v), (bytes));                                      \
+        const int fr_ = fill_u32(fst, (p), (v), (bytes));                                              \
         if (fr_) return fr_;                                                                           \
     } while (0)
 
@@ -2551,12 +2568,12 @@ KArgs base_args(galn_hip_readset *rs, uint32_t flags)
     a.out_rec = rs->d_rec;
     a.out_cap = rs->rec_cap;
     a.out_top = rs->d_top;
-    a.pool_chk = idx->d_pool_chk;
-    a.pool_link = idx->d_pool_link;
-    a.gnext = idx->d_gnext;
-    a.ghead = idx->d_ghead;
-    a.gbump = idx->d_gbump;
-    a.n_gran = idx->n_gran;
+    a.pool_chk = rs->ctx->d_pool_chk;
+    a.pool_link = rs->ctx->d_pool_link;
+    a.gnext = rs->ctx->d_gnext;
+    a.ghead = rs->ctx->d_ghead;
+    a.gbump = rs->ctx->d_gbump;
+    a.n_gran = rs->ctx->n_gran;
     // keeping the exact child in registers relies on children never scoring below their parent
     bool nonneg = true;
     for (const auto &o : rs->h_bopts) nonneg = nonneg && o.s_mm >= 0 && o.s_gapo >= 0 && o.s_gape >= 0;
@@ -2573,16 +2590,16 @@ int prepare_tier(galn_hip_readset *rs, const Tier &t, int slots)
     const Layout L = make_layout(rs->max_len, rs->seed_len, t.rec_cap, own_cap);
     int r = ensure_scratch(rs, (size_t)slots * L.slot_bytes);
     if (r) return r;
-    return ensure_pool(rs->idx, (uint64_t)slots * (uint64_t)(t.keep + 12));
+    return ensure_pool(rs->ctx, (uint64_t)slots * (uint64_t)(t.keep + 12));
 }
 
 // The allocations of a wave-engine launch of `waves` waves.
 int prepare_heavy(galn_hip_readset *rs, int waves, int rec_cap)
 {
-    galn_hip_index *idx = rs->idx;
-    int r = ensure_pool(idx, (uint64_t)waves * 256);
+    SearchCtx *cx = rs->ctx;
+    int r = ensure_pool(cx, (uint64_t)waves * 256);
     if (r) return r;
-    const int own_cap = (int)std::min<uint64_t>(idx->n_gran, 32768);
+    const int own_cap = (int)std::min<uint64_t>(cx->n_gran, 32768);
     const HLayout L = make_hlayout(rec_cap, own_cap);
     if (rs->hscratch_bytes < (size_t)waves * L.slot_bytes) {
         if (rs->d_hscratch) HIP_OK(hipFree(rs->d_hscratch));
@@ -2620,14 +2637,13 @@ int prepare_hand(galn_hip_readset *rs)
 // What a captured search graph holds: when any of it changes, the graph is captured again.
 void fill_key_ptrs(const galn_hip_readset *rs, GraphKey &k)
 {
-    const galn_hip_index *idx = rs->idx;
     const void *p[] = {rs->d_query, rs->d_qoff, rs->d_qlen, rs->d_maxdiff, rs->d_batch, rs->d_nstacks, rs->d_bopts,
                        rs->d_sorted, rs->d_out_n, rs->d_out_off, rs->d_pops, rs->d_trace, rs->d_bytes, rs->d_rec,
-                       rs->d_fail[0], rs->d_fail[1], rs->d_heavy, rs->d_scratch, rs->d_hscratch, idx->d_pool_chk};
+                       rs->d_fail[0], rs->d_fail[1], rs->d_heavy, rs->d_scratch, rs->d_hscratch, rs->ctx->d_pool_chk};
     static_assert(sizeof(p) == sizeof(k.ptr), "graph key pointers");
     memcpy(k.ptr, p, sizeof p);
     k.rec_cap_out = rs->rec_cap;
-    k.n_gran = idx->n_gran;
+    k.n_gran = rs->ctx->n_gran;
     k.scratch_bytes = rs->scratch_bytes;
     k.hscratch_bytes = rs->hscratch_bytes;
     k.max_len = rs->max_len, k.seed_len = rs->seed_len, k.max_stacks = rs->max_stacks, k.max_entries = rs->max_entries;
@@ -2637,7 +2653,8 @@ void fill_key_ptrs(const galn_hip_readset *rs, GraphKey &k)
 int run_tier(galn_hip_readset *rs, uint32_t flags, const Tier &t, const int32_t *list, int list_base, int n_list,
              bool persist, int slots, int fo, int block, int tier_id, uint64_t pop_budget = 0)
 {
-    galn_hip_index *idx = rs->idx;
+    SearchCtx *cx = rs->ctx;
+    hipStream_t fst = cx->stream;
     if (n_list <= 0) return 0;
     if (!persist) slots = n_list;
     slots = std::max(1, std::min(slots, n_list));
@@ -2671,8 +2688,8 @@ int run_tier(galn_hip_readset *rs, uint32_t flags, const Tier &t, const int32_t 
     a.hand_cap = (pop_budget && rs->d_hand) ? rs->hand_cap : 0;
     if (persist) FILL(rs->d_ctr + 2, 0, sizeof(int32_t));
     // an empty free stack (tag and granule all ones) and no granule handed out yet
-    FILL(idx->d_ghead, 0xffffffffu, NSHARD * 8);
-    FILL(idx->d_gbump, 0, 4);
+    FILL(cx->d_ghead, 0xffffffffu, NSHARD * 8);
+    FILL(cx->d_gbump, 0, 4);
     const int li = (int)rs->launches.size();
     if (li >= galn_hip_readset::MAX_LAUNCH) return fail(GALN_HIP_E_DEVICE, "too many launches in one search");
     while ((int)rs->lev.size() < 2 * (li + 1)) {
@@ -2683,10 +2700,10 @@ int run_tier(galn_hip_readset *rs, uint32_t flags, const Tier &t, const int32_t 
     a.algo_bytes = rs->d_algo + li;
     rs->launches.push_back({tier_id, li, 0.0, 0});
     const int grid = (slots + block - 1) / block;
-    if (!rs->capturing) HIP_OK(hipEventRecord(rs->lev[2 * li], idx->stream));
-    r = launch(idx, a, block, grid, persist, (flags & GALN_HIP_TRACE) != 0, buckets_of(rs));
+    if (!rs->capturing) HIP_OK(hipEventRecord(rs->lev[2 * li], fst));
+    r = launch(fst, a, block, grid, persist, (flags & GALN_HIP_TRACE) != 0, buckets_of(rs));
     if (r) return r;
-    if (!rs->capturing) HIP_OK(hipEventRecord(rs->lev[2 * li + 1], idx->stream));
+    if (!rs->capturing) HIP_OK(hipEventRecord(rs->lev[2 * li + 1], fst));
     return 0;
 }
 
@@ -2707,12 +2724,14 @@ int run_heavy(galn_hip_readset *rs, uint32_t flags, const int32_t *list, int n, 
               int waves)
 {
     galn_hip_index *idx = rs->idx;
+    SearchCtx *cx = rs->ctx;
+    hipStream_t fst = cx->stream;
     const bool dev_count = n < 0;  // the count is the heavy-list length k_search leaves on the device
     if (n == 0) return 0;
     waves = std::max(1, dev_count ? waves : std::min(waves, n));
     int r = prepare_heavy(rs, waves, rec_cap);
     if (r) return r;
-    const int own_cap = (int)std::min<uint64_t>(idx->n_gran, 32768);
+    const int own_cap = (int)std::min<uint64_t>(cx->n_gran, 32768);
     const HLayout L = make_hlayout(rec_cap, own_cap);
     const int wlen = rs->max_len + 1;
     const int swlen = (rs->seed_len > 0 && rs->seed_len < rs->max_len) ? rs->seed_len + 1 : 1;
@@ -2744,11 +2763,11 @@ int run_heavy(galn_hip_readset *rs, uint32_t flags, const int32_t *list, int n, 
     a.slog_off = L.slog_off;
     a.own_cap = own_cap;
     a.rec_cap = rec_cap;
-    a.pool = idx->d_pool_chk;
-    a.gnext = idx->d_gnext;
-    a.ghead = idx->d_ghead;
-    a.gbump = idx->d_gbump;
-    a.n_gran = idx->n_gran;
+    a.pool = cx->d_pool_chk;
+    a.gnext = cx->d_gnext;
+    a.ghead = cx->d_ghead;
+    a.gbump = cx->d_gbump;
+    a.n_gran = cx->n_gran;
     a.out_n = rs->d_out_n;
     a.out_off = rs->d_out_off;
     a.out_pops = rs->d_pops;
@@ -2766,8 +2785,8 @@ int run_heavy(galn_hip_readset *rs, uint32_t flags, const int32_t *list, int n, 
     a.swlen = swlen;
     a.stats = (flags & GALN_HIP_STATS) ? rs->d_stats : nullptr;
     FILL(rs->d_ctr + 2, 0, sizeof(int32_t));
-    FILL(idx->d_ghead, 0xffffffffu, NSHARD * 8);
-    FILL(idx->d_gbump, 0, 4);
+    FILL(cx->d_ghead, 0xffffffffu, NSHARD * 8);
+    FILL(cx->d_gbump, 0, 4);
     const int li = (int)rs->launches.size();
     if (li >= galn_hip_readset::MAX_LAUNCH) return fail(GALN_HIP_E_DEVICE, "too many launches in one search");
     while ((int)rs->lev.size() < 2 * (li + 1)) {
@@ -2777,7 +2796,7 @@ int run_heavy(galn_hip_readset *rs, uint32_t flags, const int32_t *list, int n, 
     }
     a.algo_bytes = rs->d_algo + li;
     rs->launches.push_back({tier_id, li, 0.0, 0});
-    if (!rs->capturing) HIP_OK(hipEventRecord(rs->lev[2 * li], idx->stream));
+    if (!rs->capturing) HIP_OK(hipEventRecord(rs->lev[2 * li], fst));
     if (a.hand && a.hand_cap > 0) {
         WArgs w{};
         w.bwt = a.bwt;
@@ -2796,15 +2815,15 @@ int run_heavy(galn_hip_readset *rs, uint32_t flags, const int32_t *list, int n, 
         w.wlen = wlen;
         w.swlen = swlen;
         const int wgrid = dev_count ? idx->n_cu * 8 : std::max(1, std::min(idx->n_cu * 8, (n + 255) / 256));
-        hipLaunchKernelGGL(k_widths, dim3(wgrid), dim3(256), 0, idx->stream, w);
+        hipLaunchKernelGGL(k_widths, dim3(wgrid), dim3(256), 0, fst, w);
         HIP_OK(hipGetLastError());
     }
     if (flags & GALN_HIP_TRACE)
-        hipLaunchKernelGGL(k_heavy<true>, dim3(waves), dim3(HW), lds, idx->stream, a);
+        hipLaunchKernelGGL(k_heavy<true>, dim3(waves), dim3(HW), lds, fst, a);
     else
-        hipLaunchKernelGGL(k_heavy<false>, dim3(waves), dim3(HW), lds, idx->stream, a);
+        hipLaunchKernelGGL(k_heavy<false>, dim3(waves), dim3(HW), lds, fst, a);
     HIP_OK(hipGetLastError());
-    if (!rs->capturing) HIP_OK(hipEventRecord(rs->lev[2 * li + 1], idx->stream));
+    if (!rs->capturing) HIP_OK(hipEventRecord(rs->lev[2 * li + 1], fst));
     return 0;
 }
 
@@ -2822,29 +2841,29 @@ int heavy_waves(galn_hip_readset *rs, uint32_t flags)
 
 int read_ctr(galn_hip_readset *rs, int which, int32_t *v)
 {
-    HIP_OK(hipMemcpyAsync(v, rs->d_ctr + which, sizeof(int32_t), hipMemcpyDeviceToHost, rs->idx->stream));
-    HIP_OK(hipStreamSynchronize(rs->idx->stream));
+    HIP_OK(hipMemcpyAsync(v, rs->d_ctr + which, sizeof(int32_t), hipMemcpyDeviceToHost, rs->ctx->stream));
+    HIP_OK(hipStreamSynchronize(rs->ctx->stream));
     return 0;
 }
 
 int grow_records(galn_hip_readset *rs)
 {
-    galn_hip_index *idx = rs->idx;
+    hipStream_t st = rs->ctx->stream;
     unsigned long long top = 0;
-    HIP_OK(hipMemcpyAsync(&top, rs->d_top, sizeof(top), hipMemcpyDeviceToHost, idx->stream));
-    HIP_OK(hipStreamSynchronize(idx->stream));
+    HIP_OK(hipMemcpyAsync(&top, rs->d_top, sizeof(top), hipMemcpyDeviceToHost, st));
+    HIP_OK(hipStreamSynchronize(st));
     const uint64_t used = std::min<uint64_t>(top, rs->rec_cap);
     const uint64_t ncap = std::max<uint64_t>(rs->rec_cap * 2, top + 4096);
     galn_aln_rec_t *nrec = nullptr;
     HIP_OK(hipMalloc(&nrec, ncap * sizeof(galn_aln_rec_t)));
-    if (used) HIP_OK(hipMemcpyAsync(nrec, rs->d_rec, used * sizeof(galn_aln_rec_t), hipMemcpyDeviceToDevice, idx->stream));
-    HIP_OK(hipStreamSynchronize(idx->stream));
+    if (used) HIP_OK(hipMemcpyAsync(nrec, rs->d_rec, used * sizeof(galn_aln_rec_t), hipMemcpyDeviceToDevice, st));
+    HIP_OK(hipStreamSynchronize(st));
     HIP_OK(hipFree(rs->d_rec));
     rs->d_rec = nrec;
     rs->rec_cap = ncap;
     unsigned long long u = used;
-    HIP_OK(hipMemcpyAsync(rs->d_top, &u, sizeof(u), hipMemcpyHostToDevice, idx->stream));
-    HIP_OK(hipStreamSynchronize(idx->stream));
+    HIP_OK(hipMemcpyAsync(rs->d_top, &u, sizeof(u), hipMemcpyHostToDevice, st));
+    HIP_OK(hipStreamSynchronize(st));
     return 0;
 }
 
@@ -2853,7 +2872,7 @@ int grow_records(galn_hip_readset *rs)
 int run_retries(galn_hip_readset *rs, uint32_t flags, int fi, int32_t n, Tier t, int slots, uint64_t budget,
                 int hwaves)
 {
-    galn_hip_index *idx = rs->idx;
+    hipStream_t fst = rs->ctx->stream;
     const int block = pick_block(rs);
     for (int round = 0; n > 0; ++round) {
         if (round >= 24) return fail(GALN_HIP_E_NOMEM, "reads still failing after %d re-runs", round);
@@ -2891,6 +2910,7 @@ int readset_search(galn_hip_readset *rs, int form, uint32_t flags, const galn_hi
     if (!tune && rs->has_tuned) tune = &rs->tuned;
     galn_hip_index *idx = rs->idx;
     HIP_OK(hipSetDevice(idx->device));
+    hipStream_t fst = rs->ctx->stream;
     rs->last_flags = flags;
     rs->launches.clear();
     const bool trace = (flags & GALN_HIP_TRACE) != 0;
@@ -2954,7 +2974,7 @@ int readset_search(galn_hip_readset *rs, int form, uint32_t flags, const galn_hi
                 }
                 // two graphs, so that the timing events sit between them rather than inside
                 auto capture = [&](int part, hipGraphExec_t *out) -> int {
-                    HIP_OK(hipStreamBeginCapture(idx->stream, hipStreamCaptureModeThreadLocal));
+                    HIP_OK(hipStreamBeginCapture(fst, hipStreamCaptureModeThreadLocal));
                     rs->capturing = true;
                     int rc = 0;
                     if (part == 0) {
@@ -2965,7 +2985,7 @@ int readset_search(galn_hip_readset *rs, int form, uint32_t flags, const galn_hi
                     }
                     rs->capturing = false;
                     hipGraph_t g = nullptr;
-                    const hipError_t ec = hipStreamEndCapture(idx->stream, &g);
+                    const hipError_t ec = hipStreamEndCapture(fst, &g);
                     if (rc) {
                         if (g) (void)hipGraphDestroy(g);
                         return rc;
@@ -2985,13 +3005,13 @@ int readset_search(galn_hip_readset *rs, int form, uint32_t flags, const galn_hi
                 rs->glaunches = rs->launches;
             }
             rs->launches = rs->glaunches;
-            HIP_OK(hipEventRecord(rs->lev[0], idx->stream));
-            HIP_OK(hipGraphLaunch(rs->gexec, idx->stream));
-            HIP_OK(hipEventRecord(rs->lev[1], idx->stream));
+            HIP_OK(hipEventRecord(rs->lev[0], fst));
+            HIP_OK(hipGraphLaunch(rs->gexec, fst));
+            HIP_OK(hipEventRecord(rs->lev[1], fst));
             if (rs->gexec2) {
-                HIP_OK(hipEventRecord(rs->lev[2], idx->stream));
-                HIP_OK(hipGraphLaunch(rs->gexec2, idx->stream));
-                HIP_OK(hipEventRecord(rs->lev[3], idx->stream));
+                HIP_OK(hipEventRecord(rs->lev[2], fst));
+                HIP_OK(hipGraphLaunch(rs->gexec2, fst));
+                HIP_OK(hipEventRecord(rs->lev[3], fst));
             }
         } else {
             r = run_tier(rs, flags, t0, list, 0, rs->n, true, slots, 0, block, 0, budget);
@@ -3004,7 +3024,7 @@ int readset_search(galn_hip_readset *rs, int form, uint32_t flags, const galn_hi
     int32_t nf = 0;
     if ((r = read_ctr(rs, 0, &nf))) return r;
     if (nf > 0 && (r = run_retries(rs, flags, 0, nf, t0, slots, budget, std::max(1, hwaves)))) return r;
-    HIP_OK(hipStreamSynchronize(idx->stream));
+    HIP_OK(hipStreamSynchronize(fst));
     std::vector<unsigned long long> ab(rs->launches.size());
     if (!ab.empty())
         HIP_OK(hipMemcpy(ab.data(), rs->d_algo, ab.size() * sizeof(unsigned long long), hipMemcpyDeviceToHost));
@@ -3039,7 +3059,7 @@ int readset_reserve(galn_hip_readset *rs, int n, uint64_t qbytes, int nb)
 {
     const bool grow_n = n > rs->cap_n, grow_q = qbytes > rs->cap_q, grow_b = nb > rs->cap_b;
     if (!grow_n && !grow_q && !grow_b) return 0;
-    HIP_OK(hipStreamSynchronize(rs->idx->stream));
+    HIP_OK(hipStreamSynchronize(rs->ctx->stream));
     auto regrow = [](void **p, size_t bytes) __attribute__((always_inline)) -> bool {
         if (*p) (void)hipFree(*p);
         *p = nullptr;
@@ -3102,11 +3122,17 @@ int readset_fill(galn_hip_readset *rs, const galn_hip_reads *reads)
     rs->h_bopts.assign(reads->bopts, reads->bopts + reads->n_batches);
     rs->max_entries = 0;
     for (const auto &o : rs->h_bopts) rs->max_entries = std::max(rs->max_entries, std::max(o.max_entries, 0));
-    // claim order of the batched form: longest reads first (their searches are the longest)
+    // claim order of the batched form: longest reads first (their searches are the longest); a
+    // counting sort by length (stable; lengths above the kernel limit share the top bin)
     std::vector<int32_t> order(n);
-    std::iota(order.begin(), order.end(), 0);
-    std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return reads->qlen[x] > reads->qlen[y]; });
-    hipStream_t s = rs->idx->stream;
+    {
+        std::vector<int32_t> cnt(MAXLEN + 3, 0);
+        auto bin = [&](int i) { return MAXLEN + 1 - std::min(std::max(reads->qlen[i], 0), MAXLEN + 1); };
+        for (int i = 0; i < n; ++i) ++cnt[bin(i) + 1];
+        for (int b = 1; b <= MAXLEN + 2; ++b) cnt[b] += cnt[b - 1];
+        for (int i = 0; i < n; ++i) order[cnt[bin(i)]++] = i;
+    }
+    hipStream_t s = rs->ctx->stream;
     if (hipMemcpyAsync(rs->d_query, reads->query, qbytes, hipMemcpyHostToDevice, s) != hipSuccess ||
         hipMemcpyAsync(rs->d_qoff, reads->qoff, (size_t)n * 8, hipMemcpyHostToDevice, s) != hipSuccess ||
         hipMemcpyAsync(rs->d_qlen, reads->qlen, (size_t)n * 4, hipMemcpyHostToDevice, s) != hipSuccess ||
@@ -3151,6 +3177,28 @@ thread_local bool tl_server_worker = false;
 __attribute__((visibility("hidden"))) void galn_hip_server_worker_mark() { tl_server_worker = true; }
 __attribute__((visibility("hidden"))) void galn_hip_index_serve(galn_hip_index *idx, int delta) { idx->n_servers += delta; }
 
+// A server slot's read set gets a stream and a stack pool of its own, so that the slots' launches
+// run at the same time (the read set was made on the index's context).
+__attribute__((visibility("hidden"))) int galn_hip_readset_make_private(galn_hip_readset *rs)
+{
+    if (rs->own_ctx) return 0;
+    HIP_OK(hipSetDevice(rs->idx->device));
+    HIP_OK(hipStreamSynchronize(rs->ctx->stream));
+    SearchCtx *c = new SearchCtx;
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete c;
+        return fail(GALN_HIP_E_DEVICE, "readset: stream creation failed");
+    }
+    // graphs captured on the old stream hold the old pool: captured again on first use
+    for (hipGraphExec_t *ge : {&rs->gexec, &rs->gexec2}) {
+        if (*ge) (void)hipGraphExecDestroy(*ge);
+        *ge = nullptr;
+    }
+    rs->own_ctx = c;
+    rs->ctx = c;
+    return 0;
+}
+
 namespace {
 int noalloc_setup(int device);
 
@@ -3193,20 +3241,20 @@ int galn_hip_index_open(const uint32_t *bwt, const galn_bwt_meta_t *meta, int de
         delete idx;
         return fail(GALN_HIP_E_NOMEM, "index of %zu bytes does not fit in device memory", bytes);
     }
-    if (hipStreamCreateWithFlags(&idx->stream, hipStreamNonBlocking) != hipSuccess ||
-        hipMemsetAsync(idx->d_bwt, 0, alloc, idx->stream) != hipSuccess ||
-        hipMemcpyAsync(idx->d_bwt, bwt, bytes, hipMemcpyHostToDevice, idx->stream) != hipSuccess) {
-        if (idx->stream) (void)hipStreamDestroy(idx->stream);
+    if (hipStreamCreateWithFlags(&idx->ctx0.stream, hipStreamNonBlocking) != hipSuccess ||
+        hipMemsetAsync(idx->d_bwt, 0, alloc, idx->ctx0.stream) != hipSuccess ||
+        hipMemcpyAsync(idx->d_bwt, bwt, bytes, hipMemcpyHostToDevice, idx->ctx0.stream) != hipSuccess) {
+        if (idx->ctx0.stream) (void)hipStreamDestroy(idx->ctx0.stream);
         (void)hipFree(idx->d_bwt);
         delete idx;
         return fail(GALN_HIP_E_DEVICE, "index upload failed");
     }
     const uint64_t n_blocks = ((uint64_t)meta->bwt_size + 15) / 16;
     if (n_blocks > 0)
-        hipLaunchKernelGGL(k_planes, dim3((unsigned)((n_blocks + 255) / 256)), dim3(256), 0, idx->stream, idx->d_bwt,
+        hipLaunchKernelGGL(k_planes, dim3((unsigned)((n_blocks + 255) / 256)), dim3(256), 0, idx->ctx0.stream, idx->d_bwt,
                            n_blocks);
-    if (hipGetLastError() != hipSuccess || hipStreamSynchronize(idx->stream) != hipSuccess) {
-        (void)hipStreamDestroy(idx->stream);
+    if (hipGetLastError() != hipSuccess || hipStreamSynchronize(idx->ctx0.stream) != hipSuccess) {
+        (void)hipStreamDestroy(idx->ctx0.stream);
         (void)hipFree(idx->d_bwt);
         delete idx;
         return fail(GALN_HIP_E_DEVICE, "index upload failed");
@@ -3225,12 +3273,11 @@ int galn_hip_index_close(galn_hip_index *idx)
         return fail(GALN_HIP_E_BUSY, "index_close: %d read sets (or servers) of this index are still open",
                     idx->n_readsets.load());
     (void)hipSetDevice(idx->device);
-    if (idx->stream) (void)hipStreamSynchronize(idx->stream);
+    if (idx->ctx0.stream) (void)hipStreamSynchronize(idx->ctx0.stream);
     (void)hipFree(idx->d_bwt);
-    void *pool[] = {idx->d_pool_chk, idx->d_pool_link, idx->d_gnext, idx->d_gbump, idx->d_ghead, idx->d_sa};
-    for (void *p : pool)
-        if (p) (void)hipFree(p);
-    if (idx->stream) (void)hipStreamDestroy(idx->stream);
+    if (idx->d_sa) (void)hipFree(idx->d_sa);
+    idx->ctx0.free_pool();
+    if (idx->ctx0.stream) (void)hipStreamDestroy(idx->ctx0.stream);
     delete idx;
     return 0;
 }
@@ -3241,7 +3288,7 @@ int galn_hip_readset_free(galn_hip_readset *rs)
 {
     if (!rs) return 0;
     (void)hipSetDevice(rs->idx->device);
-    (void)hipStreamSynchronize(rs->idx->stream);
+    (void)hipStreamSynchronize(rs->ctx->stream);
     void *ptrs[] = {rs->d_query, rs->d_qoff, rs->d_qlen, rs->d_maxdiff, rs->d_batch, rs->d_nstacks, rs->d_bopts,
                     rs->d_sorted, rs->d_out_n, rs->d_out_off, rs->d_pops, rs->d_trace, rs->d_bytes, rs->d_rec, rs->d_top,
                     rs->d_fail[0], rs->d_fail[1], rs->d_ctr, rs->d_scratch, rs->d_algo, rs->d_heavy, rs->d_hscratch,
@@ -3253,6 +3300,11 @@ int galn_hip_readset_free(galn_hip_readset *rs)
     for (auto &e : rs->lev) (void)hipEventDestroy(e);
     if (rs->gexec) (void)hipGraphExecDestroy(rs->gexec);
     if (rs->gexec2) (void)hipGraphExecDestroy(rs->gexec2);
+    if (rs->own_ctx) {
+        rs->own_ctx->free_pool();
+        if (rs->own_ctx->stream) (void)hipStreamDestroy(rs->own_ctx->stream);
+        delete rs->own_ctx;
+    }
     rs->idx->n_readsets.fetch_sub(1);
     delete rs;
     return 0;
@@ -3268,6 +3320,7 @@ int galn_hip_readset_upload(galn_hip_index *idx, const galn_hip_reads *reads, ga
     HIP_OK(hipSetDevice(idx->device));
     galn_hip_readset *rs = new galn_hip_readset;
     rs->idx = idx;
+    rs->ctx = &idx->ctx0;
     idx->n_readsets.fetch_add(1);
     auto M = [&](void **p, size_t b) __attribute__((always_inline)) -> bool { return hipMalloc(p, b) == hipSuccess; };
     bool ok = M((void **)&rs->d_top, 8) && M((void **)&rs->d_ctr, 8 * sizeof(int32_t)) && M((void **)&rs->d_stats, 16 * 8) &&
@@ -3391,7 +3444,7 @@ int galn_hip_readset_results(galn_hip_readset *rs, galn_hip_out *out)
         out->n_rec = 0;
         return 0;
     }
-    hipStream_t s = idx->stream;
+    hipStream_t s = rs->ctx->stream;
     // records in read order, arranged on the device: offsets = prefix sums of the record counts
     // (errors count 0), then every read's records gathered from the arena
     size_t tmp_bytes = 0;
@@ -3526,7 +3579,7 @@ int galn_hip_sa_positions(galn_hip_index *idx, const uint64_t *rows, uint64_t n,
     HIP_OK(hipMalloc(&d, chunk * (ref_len ? 24 : 16)));
     int32_t *const d_rl = ref_len ? reinterpret_cast<int32_t *>(d + 2 * chunk) : nullptr;
     int32_t *const d_st = ref_len ? d_rl + chunk : nullptr;
-    hipStream_t s = idx->stream;
+    hipStream_t s = idx->ctx0.stream;
     int rc = 0;
     for (uint64_t a = 0; a < n && !rc; a += chunk) {
         const uint64_t m = std::min(chunk, n - a);
@@ -3687,7 +3740,7 @@ int noalloc_search(galn_hip_index *idx, const galn_hip_reads *reads, galn_hip_ou
     int32_t *const d_nstacks = reinterpret_cast<int32_t *>(S + NL.nstacks);
     int32_t *const d_fail[2] = {reinterpret_cast<int32_t *>(S + NL.fail0), reinterpret_cast<int32_t *>(S + NL.fail1)};
     uint64_t *const d_pops = out->n_pops ? out->n_pops : reinterpret_cast<uint64_t *>(S + NL.pops);
-    hipStream_t st = idx->stream;
+    hipStream_t st = idx->ctx0.stream, fst = st;
 
     // per-batch options in, counters cleared (fill kernels: the array keeps the last call's contents)
     const int nb = reads->n_batches;
@@ -3783,7 +3836,7 @@ int noalloc_search(galn_hip_index *idx, const galn_hip_reads *reads, galn_hip_ou
         FILL(ctr + 4, 0, 3 * sizeof(int32_t));
         FILL(ghead, 0xffffffffu, NSHARD * 8);
         FILL(gbump, 0, 4);
-        r = launch(idx, a, block, (lanes + block - 1) / block, true, trace, buckets);
+        r = launch(st, a, block, (lanes + block - 1) / block, true, trace, buckets);
         if (r) return r;
         int32_t c[8];
         HIP_OK(hipMemcpyAsync(c, ctr, sizeof c, hipMemcpyDeviceToHost, st));

@@ -1,13 +1,21 @@
 // galn_server.cpp -- serving loop: one resident index, requests queued from any thread,
-// searched by one worker thread in shared launches.
+// searched in shared launches by several launch slots at once.
 //
 // The reference has no serving mode (it is a one-shot CLI, src/galn.cu:146-178); this is the
 // request front end for the same search. A request is a galn_hip_reads with its own options.
-// Each launch takes, in arrival order, up to `quantum_reads` reads from every request that
-// still has reads to search, then tops the launch up to `max_launch_reads` from the oldest
-// requests; each request contributes one contiguous range of its reads per launch, so its
-// records are appended in read order. A read's search depends only on the read, its batch
-// options and the index, so its records do not depend on what it shares a launch with.
+//
+// Launch slots: each slot is a worker thread with its own device read set, stream and stack pool,
+// so up to `slots` launches are on the device together. A launch is bounded by its longest
+// search (a read that backtracks millions of entries keeps one wave busy while the rest of the
+// GPU has finished), and the next slot's launch fills the device meanwhile: requests that arrive
+// during a launch's tail start at once instead of waiting for it.
+//
+// Each launch takes, in arrival order, up to `quantum_reads` reads from every request that still
+// has reads to search, then tops the launch up to `max_launch_reads` from the oldest requests. A
+// request's reads may be in several launches at once; each launch's records go to the request
+// by read index, so the request's records are in read order whichever launch ends first. A
+// read's search depends only on the read, its batch options and the index, so its records do
+// not depend on what it shares a launch with.
 #include <algorithm>
 #include <chrono>
 #include <condition_variable>
@@ -39,14 +47,17 @@ struct Request {
     std::vector<galn_search_opt_t> bopts;
     uint64_t footprint = 0;
     Clock::time_point t_submit, t_done;
+    // scheduling state (under the server lock)
     int next = 0;       // first read not yet given to a launch
-    int finished = 0;   // reads whose launch has completed
+    int inflight = 0;   // launches holding reads of this request
+    int done_reads = 0; // reads whose launch has completed
     int status = 0;     // 0 pending, 1 finished, < 0 library error of a launch
+    bool done = false;  // no launch touches the request any more: wait() and result() may go on
     std::string err;    // text of that error
-    // results, in read order
+    // results by read (each read written by the one launch that searched it)
     std::vector<int32_t> n_aln;
-    std::vector<uint64_t> rec_off, pops, trace, bytes;
-    std::vector<galn_aln_rec_t> rec;
+    std::vector<uint64_t> pops, trace, bytes;
+    std::map<int, std::vector<galn_aln_rec_t>> recs;  // records of the reads from each start index on
 };
 
 struct Piece {
@@ -66,11 +77,12 @@ uint64_t footprint_of(const galn_hip_reads *r, uint64_t qbytes)
 
 // galn_search.hip: this thread's galn_hip_last_error text; the open-handle count of an index;
 // the count of servers holding an index, and the mark of their worker threads (only those may
-// search an index that a server holds)
+// search an index that a server holds); a slot read set's own stream and stack pool
 void galn_hip_set_error(const char *msg);
 void galn_hip_index_hold(galn_hip_index *idx, int delta);
 void galn_hip_index_serve(galn_hip_index *idx, int delta);
 void galn_hip_server_worker_mark();
+int galn_hip_readset_make_private(galn_hip_readset *rs);
 
 namespace {
 
@@ -97,9 +109,10 @@ struct galn_hip_server {
     uint64_t next_id = 1;
     uint64_t queued_bytes = 0;
     bool stop = false;
-    std::thread worker;
-    galn_hip_readset *rs = nullptr;
+    std::vector<std::thread> workers;
+    std::vector<galn_hip_readset *> slot_rs;
     double stats[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    int busy_slots = 0;
     // GALN_HIP_TEST_SERVER_FAIL=n: the n-th launch (from 1) fails on the host before it reaches
     // the device, so that tests can drive the error path of a request spread over several launches
     long fail_at = 0;
@@ -125,14 +138,44 @@ void schedule(galn_hip_server *S, std::vector<Piece> &pieces)
         p.count += more;
         room -= more;
     }
-    for (Piece &p : pieces) p.q->next = p.start + p.count;
-    while (!S->waiting.empty() && S->waiting.front()->next == S->waiting.front()->n) S->waiting.pop_front();
-    // requests further back may also be fully given out
+    for (Piece &p : pieces) {
+        p.q->next = p.start + p.count;
+        p.q->inflight += 1;
+    }
     S->waiting.erase(std::remove_if(S->waiting.begin(), S->waiting.end(), [](Request *q) { return q->next == q->n; }),
                      S->waiting.end());
 }
 
-void worker_main(galn_hip_server *S)
+// A launch of `pieces` has ended with code r (caller holds the lock).
+void complete(galn_hip_server *S, const std::vector<Piece> &pieces, int r, const std::string &err,
+              std::vector<std::vector<galn_aln_rec_t>> &piece_recs, Clock::time_point t1)
+{
+    for (size_t k = 0; k < pieces.size(); ++k) {
+        const Piece &p = pieces[k];
+        Request &q = *p.q;
+        q.inflight -= 1;
+        q.done_reads += p.count;
+        if (r == 0) {
+            q.recs[p.start] = std::move(piece_recs[k]);
+        } else if (q.status == 0) {
+            q.status = r;
+            q.err = err;
+        }
+        if (q.status < 0 && q.next < q.n) {
+            // a failed request gets no further launches
+            S->waiting.erase(std::remove(S->waiting.begin(), S->waiting.end(), &q), S->waiting.end());
+            q.next = q.n;
+        }
+        if (q.status == 0 && q.done_reads == q.n) q.status = 1;
+        if (!q.done && q.inflight == 0 && (q.status < 0 || q.done_reads == q.n)) {
+            q.done = true;
+            q.t_done = t1;
+            S->stats[2] += 1;
+        }
+    }
+}
+
+void worker_main(galn_hip_server *S, int slot)
 {
     galn_hip_server_worker_mark();
     std::vector<Piece> pieces;
@@ -141,29 +184,38 @@ void worker_main(galn_hip_server *S)
     std::vector<int32_t> qlen, max_diff, batch, n_stacks, n_aln;
     std::vector<galn_search_opt_t> bopts;
     std::vector<galn_aln_rec_t> rec(1 << 16);
+    std::vector<std::vector<galn_aln_rec_t>> piece_recs;
     const bool want_trace = (S->cfg.flags & GALN_HIP_TRACE) != 0;
+    galn_hip_readset *&rs = S->slot_rs[slot];
     for (;;) {
+        long launch_no;
         {
             std::unique_lock<std::mutex> lk(S->mu);
             S->cv_work.wait(lk, [&] { return S->stop || !S->waiting.empty(); });
             if (S->waiting.empty()) return;  // stop requested and nothing left
             schedule(S, pieces);
+            launch_no = ++S->n_launch;
+            S->busy_slots += 1;
+            S->stats[4] = std::max(S->stats[4], (double)S->busy_slots);
         }
-        // the merged read set of this launch (the pieces' requests are not changed by
-        // other threads while their reads are in flight)
+        // the merged read set of this launch (the reads of a scheduled piece are not changed by
+        // other threads while they are in flight)
         query.clear(), qoff.clear(), qlen.clear(), max_diff.clear(), batch.clear(), n_stacks.clear(), bopts.clear();
         for (const Piece &p : pieces) {
+            // a request keeps its queries back to back, so a piece's are one range
             const Request &q = *p.q;
-            const int b0 = (int)bopts.size();
+            const int b0 = (int)bopts.size(), e = p.start + p.count;
             bopts.insert(bopts.end(), q.bopts.begin(), q.bopts.end());
             n_stacks.insert(n_stacks.end(), q.n_stacks.begin(), q.n_stacks.end());
-            for (int i = p.start; i < p.start + p.count; ++i) {
-                qoff.push_back(query.size());
-                query.insert(query.end(), q.query.begin() + q.qoff[i], q.query.begin() + q.qoff[i] + q.qlen[i]);
-                qlen.push_back(q.qlen[i]);
-                max_diff.push_back(q.max_diff[i]);
+            const uint64_t q0 = q.qoff[p.start], q1 = q.qoff[e - 1] + (uint64_t)q.qlen[e - 1];
+            const uint64_t base = query.size();
+            query.insert(query.end(), q.query.begin() + q0, q.query.begin() + q1);
+            for (int i = p.start; i < e; ++i) {
+                qoff.push_back(base + q.qoff[i] - q0);
                 batch.push_back(b0 + q.batch[i]);
             }
+            qlen.insert(qlen.end(), q.qlen.begin() + p.start, q.qlen.begin() + e);
+            max_diff.insert(max_diff.end(), q.max_diff.begin() + p.start, q.max_diff.begin() + e);
         }
         const int m = (int)qlen.size();
         galn_hip_reads R{};
@@ -178,10 +230,16 @@ void worker_main(galn_hip_server *S)
         R.n_stacks = n_stacks.data();
         const auto t0 = Clock::now();
         int r = 0;
-        if (++S->n_launch == S->fail_at)
-            r = srv_fail(GALN_HIP_E_DEVICE, "launch %ld: failure injected by GALN_HIP_TEST_SERVER_FAIL", S->n_launch);
-        if (!r) r = S->rs ? galn_hip_readset_reload(S->rs, &R) : galn_hip_readset_upload(S->idx, &R, &S->rs);
-        if (!r) r = galn_hip_readset_search(S->idx, S->rs, S->cfg.form, S->cfg.flags, nullptr);
+        if (launch_no == S->fail_at)
+            r = srv_fail(GALN_HIP_E_DEVICE, "launch %ld: failure injected by GALN_HIP_TEST_SERVER_FAIL", launch_no);
+        if (!r) {
+            if (rs) {
+                r = galn_hip_readset_reload(rs, &R);
+            } else if (!(r = galn_hip_readset_upload(S->idx, &R, &rs))) {
+                r = galn_hip_readset_make_private(rs);
+            }
+        }
+        if (!r) r = galn_hip_readset_search(S->idx, rs, S->cfg.form, S->cfg.flags, nullptr);
         n_aln.resize(m), off.resize(m), pops.resize(m), trace.resize(m), bytes.resize(m);
         galn_hip_out O{};
         O.n_aln = n_aln.data();
@@ -192,52 +250,41 @@ void worker_main(galn_hip_server *S)
         if (!r) {
             O.rec = rec.data();
             O.rec_cap = rec.size();
-            r = galn_hip_readset_results(S->rs, &O);
+            r = galn_hip_readset_results(rs, &O);
             if (r == GALN_HIP_E_RECCAP) {
                 rec.resize(O.n_rec + O.n_rec / 4);
                 O.rec = rec.data();
                 O.rec_cap = rec.size();
-                r = galn_hip_readset_results(S->rs, &O);
+                r = galn_hip_readset_results(rs, &O);
             }
         }
         const auto t1 = Clock::now();
         const std::string err = r ? std::string(galn_hip_last_error()) : std::string();
-        // hand the records to their requests, in read order
-        int j = 0;
-        for (const Piece &p : pieces) {
-            Request &q = *p.q;
-            for (int i = p.start; i < p.start + p.count; ++i, ++j) {
-                if (r) continue;
-                q.n_aln[i] = n_aln[j];
-                q.pops[i] = pops[j];
-                q.trace[i] = want_trace ? trace[j] : 0;
-                q.bytes[i] = bytes[j];
-                q.rec_off[i] = q.rec.size();
-                if (n_aln[j] > 0) q.rec.insert(q.rec.end(), rec.begin() + off[j], rec.begin() + off[j] + n_aln[j]);
+        // results by read, and each piece's records in read order (the request's arrays are sized
+        // at submit; this launch alone writes these reads)
+        piece_recs.assign(pieces.size(), {});
+        if (!r) {
+            int j = 0;
+            for (size_t k = 0; k < pieces.size(); ++k) {
+                const Piece &p = pieces[k];
+                Request &q = *p.q;
+                // the launch's records are in read order: a piece's are one range
+                const int j0 = j;
+                uint64_t r1 = off[j0];
+                for (int i = p.start; i < p.start + p.count; ++i, ++j) {
+                    q.n_aln[i] = n_aln[j];
+                    q.pops[i] = pops[j];
+                    q.trace[i] = want_trace ? trace[j] : 0;
+                    q.bytes[i] = bytes[j];
+                    r1 = off[j] + (uint64_t)std::max(n_aln[j], 0);
+                }
+                piece_recs[k].assign(rec.begin() + off[j0], rec.begin() + r1);
             }
         }
         {
             std::lock_guard<std::mutex> lk(S->mu);
-            for (const Piece &p : pieces) {
-                Request &q = *p.q;
-                q.finished += p.count;
-                if (r && q.status == 0) {
-                    q.status = r;
-                    q.err = err;
-                }
-                if (q.status < 0 && q.next < q.n) {
-                    // a failed request gets no further launches: out of the queue before its
-                    // status is published (result() may release it as soon as the lock drops)
-                    S->waiting.erase(std::remove(S->waiting.begin(), S->waiting.end(), &q), S->waiting.end());
-                    q.next = q.n;
-                }
-                if (q.status < 0) q.finished = q.next;
-                if (q.finished == q.n && q.status == 0) q.status = 1;
-                if (q.status != 0 && q.t_done == Clock::time_point{}) {
-                    q.t_done = t1;
-                    S->stats[2] += 1;
-                }
-            }
+            complete(S, pieces, r, err, piece_recs, t1);
+            S->busy_slots -= 1;
             S->stats[0] += 1;
             S->stats[1] += m;
             S->stats[3] += std::chrono::duration<double, std::milli>(t1 - t0).count();
@@ -262,10 +309,13 @@ int galn_hip_server_open(galn_hip_index *idx, const galn_hip_server_cfg_t *cfg, 
     S->cfg.quantum_reads = std::min(S->cfg.quantum_reads, S->cfg.max_launch_reads);
     if (S->cfg.max_queued_bytes == 0) S->cfg.max_queued_bytes = 4ull << 30;
     if (S->cfg.form != 0) S->cfg.form = 1;
+    if (S->cfg.slots <= 0) S->cfg.slots = 3;
+    S->cfg.slots = std::min(S->cfg.slots, 16);
     if (const char *v = getenv("GALN_HIP_TEST_SERVER_FAIL")) S->fail_at = strtol(v, nullptr, 10);
     galn_hip_index_hold(idx, 1);
     galn_hip_index_serve(idx, 1);
-    S->worker = std::thread(worker_main, S);
+    S->slot_rs.assign(S->cfg.slots, nullptr);
+    for (int k = 0; k < S->cfg.slots; ++k) S->workers.emplace_back(worker_main, S, k);
     *out = S;
     return 0;
 }
@@ -297,7 +347,6 @@ int galn_hip_server_submit(galn_hip_server *S, const galn_hip_reads *r, uint64_t
     q->bopts.assign(r->bopts, r->bopts + r->n_batches);
     q->n_stacks.assign(r->n_stacks, r->n_stacks + r->n_batches);
     q->n_aln.assign(n, 0);
-    q->rec_off.assign(n, 0);
     q->pops.assign(n, 0);
     q->trace.assign(n, 0);
     q->bytes.assign(n, 0);
@@ -315,6 +364,7 @@ int galn_hip_server_submit(galn_hip_server *S, const galn_hip_reads *r, uint64_t
     S->reqs.emplace(q->id, std::move(q));
     if (n == 0) {
         raw->status = 1;
+        raw->done = true;
         raw->t_done = raw->t_submit;
         S->stats[2] += 1;
         lk.unlock();
@@ -334,11 +384,12 @@ int galn_hip_server_wait(galn_hip_server *S, uint64_t id, int timeout_ms)
     auto it = S->reqs.find(id);
     if (it == S->reqs.end()) return srv_fail(GALN_HIP_E_ARG, "server_wait: unknown request %llu", (unsigned long long)id);
     Request *q = it->second.get();
-    auto ready = [&] { return q->status != 0; };
+    auto ready = [&] { return q->done; };
     if (timeout_ms < 0)
         S->cv_done.wait(lk, ready);
     else if (timeout_ms > 0)
         S->cv_done.wait_for(lk, std::chrono::milliseconds(timeout_ms), ready);
+    if (!q->done) return 0;
     if (q->status < 0) galn_hip_set_error(q->err.c_str());
     return q->status > 0 ? 1 : q->status;
 }
@@ -350,21 +401,31 @@ int galn_hip_server_result(galn_hip_server *S, uint64_t id, galn_hip_out *out, d
     auto it = S->reqs.find(id);
     if (it == S->reqs.end()) return srv_fail(GALN_HIP_E_ARG, "server_result: unknown request %llu", (unsigned long long)id);
     Request *q = it->second.get();
-    if (q->status == 0 || q->finished < q->next)
-        return srv_fail(GALN_HIP_E_BUSY, "server_result: request %llu not finished", (unsigned long long)id);
+    if (!q->done) return srv_fail(GALN_HIP_E_BUSY, "server_result: request %llu not finished", (unsigned long long)id);
     int r = 0;
     if (q->status < 0) {
         r = q->status;
         galn_hip_set_error(q->err.c_str());
     } else {
-        out->n_rec = q->rec.size();
+        uint64_t total = 0;
+        for (const auto &kv : q->recs) total += kv.second.size();
+        out->n_rec = total;
         if (q->n > 0 && (!out->n_aln || !out->rec_off)) return srv_fail(GALN_HIP_E_ARG, "server_result: null array");
-        if (q->rec.size() > out->rec_cap || (!q->rec.empty() && !out->rec))
+        if (total > out->rec_cap || (total > 0 && !out->rec))
             return srv_fail(GALN_HIP_E_RECCAP, "server_result: record array holds %llu, %llu needed",
-                            (unsigned long long)out->rec_cap, (unsigned long long)q->rec.size());
+                            (unsigned long long)out->rec_cap, (unsigned long long)total);
+        // pieces in read order (they cover reads 0 .. n - 1): the records, and each read's offset
+        uint64_t at = 0;
+        for (auto pc = q->recs.begin(); pc != q->recs.end(); ++pc) {
+            const auto nx = std::next(pc);
+            const int end = nx == q->recs.end() ? q->n : nx->first;
+            std::copy(pc->second.begin(), pc->second.end(), out->rec + at);
+            for (int i = pc->first; i < end; ++i) {
+                out->rec_off[i] = at;
+                at += (uint64_t)std::max(q->n_aln[i], 0);
+            }
+        }
         std::copy(q->n_aln.begin(), q->n_aln.end(), out->n_aln);
-        std::copy(q->rec_off.begin(), q->rec_off.end(), out->rec_off);
-        std::copy(q->rec.begin(), q->rec.end(), out->rec);
         if (out->n_pops) std::copy(q->pops.begin(), q->pops.end(), out->n_pops);
         if (out->trace) std::copy(q->trace.begin(), q->trace.end(), out->trace);
         if (out->occ_bytes) std::copy(q->bytes.begin(), q->bytes.end(), out->occ_bytes);
@@ -380,6 +441,7 @@ int galn_hip_server_stats(galn_hip_server *S, double out[8])
     if (!S || !out) return srv_fail(GALN_HIP_E_ARG, "server_stats: null argument");
     std::lock_guard<std::mutex> lk(S->mu);
     std::copy(S->stats, S->stats + 8, out);
+    out[5] = S->cfg.slots;
     return 0;
 }
 
@@ -391,8 +453,10 @@ int galn_hip_server_close(galn_hip_server *S)
         S->stop = true;
     }
     S->cv_work.notify_all();
-    if (S->worker.joinable()) S->worker.join();
-    if (S->rs) galn_hip_readset_free(S->rs);
+    for (auto &w : S->workers)
+        if (w.joinable()) w.join();
+    for (galn_hip_readset *rs : S->slot_rs)
+        if (rs) galn_hip_readset_free(rs);
     galn_hip_index_serve(S->idx, -1);
     galn_hip_index_hold(S->idx, -1);
     delete S;

@@ -79,7 +79,7 @@ class Tune(C.Structure):
 
 class ServerCfg(C.Structure):
     _fields_ = [("max_launch_reads", C.c_int32), ("quantum_reads", C.c_int32), ("max_queued_bytes", C.c_uint64),
-                ("flags", C.c_uint32), ("form", C.c_int32)]
+                ("flags", C.c_uint32), ("form", C.c_int32), ("slots", C.c_int32)]
 
 
 class GapOpt(C.Structure):
@@ -534,13 +534,14 @@ class ReadSet:
 
 class Server:
     """The serving loop (galn_hip_server_*): the index stays resident; requests are queued
-    and searched in shared launches by a worker thread."""
+    and searched in shared launches, up to `slots` launches on the device at once."""
 
     def __init__(self, index: Index, max_launch_reads: int = 0, quantum_reads: int = 0, max_queued_bytes: int = 0,
-                 flags: int = 0, form: int = FORM_BATCHED):
+                 flags: int = 0, form: int = FORM_BATCHED, slots: int = 0):
         cfg = ServerCfg()
         cfg.max_launch_reads, cfg.quantum_reads = int(max_launch_reads), int(quantum_reads)
         cfg.max_queued_bytes, cfg.flags, cfg.form = int(max_queued_bytes), int(flags), int(form)
+        cfg.slots = int(slots)
         h = C.c_void_p()
         rc = hip_lib().galn_hip_server_open(index.h, C.byref(cfg), C.byref(h))
         if rc != 0:
@@ -596,7 +597,8 @@ class Server:
     def stats(self) -> dict:
         v = np.zeros(8, np.float64)
         hip_lib().galn_hip_server_stats(self.h, _p(v))
-        return {"launches": int(v[0]), "reads": int(v[1]), "requests": int(v[2]), "launch_ms": float(v[3])}
+        return {"launches": int(v[0]), "reads": int(v[1]), "requests": int(v[2]), "launch_ms": float(v[3]),
+                "most_in_flight": int(v[4]), "slots": int(v[5])}
 
     def close(self):
         if self.h:

@@ -28,7 +28,7 @@
 extern "C" {
 #endif
 
-#define GALN_HIP_ABI_VERSION 2
+#define GALN_HIP_ABI_VERSION 3
 
 /* ---- library return codes ---- */
 #define GALN_HIP_OK             0
@@ -137,6 +137,8 @@ typedef struct {
                                   GALN_HIP_E_BUSY (default 4 GiB) */
     uint32_t flags;            /* search flags of every launch */
     int32_t form;              /* 1 = batched (default), 0 = plain */
+    int32_t slots;             /* launches on the device at once, each with its own worker thread,
+                                  stream and stack pool (default 3, at most 16) */
 } galn_hip_server_cfg_t;
 
 typedef struct galn_hip_index galn_hip_index;
@@ -233,7 +235,7 @@ int galn_hip_server_submit(galn_hip_server *srv, const galn_hip_reads *reads, ui
 int galn_hip_server_wait(galn_hip_server *srv, uint64_t id, int timeout_ms);
 int galn_hip_server_result(galn_hip_server *srv, uint64_t id, galn_hip_out *out, double *latency_ms);
 /* Counters since open: launches, reads searched, requests finished, summed launch ms (host
- * clock around each launch), 0, 0, 0, 0. */
+ * clock around each launch), most launches in flight at once, slots, 0, 0. */
 int galn_hip_server_stats(galn_hip_server *srv, double out[8]);
 /* Finishes the requests already submitted, then stops the worker. */
 int galn_hip_server_close(galn_hip_server *srv);

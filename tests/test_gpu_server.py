@@ -44,8 +44,11 @@ def requests(golden_dir):
     return w, out
 
 
-def test_requests_in_a_mix_equal_alone_and_reference(golden_dir):
-    print("case server mix: 9 requests, 2 option sets", flush=True)
+@pytest.mark.parametrize("slots", [1, 4])
+def test_requests_in_a_mix_equal_alone_and_reference(slots, golden_dir):
+    """With one launch slot, and with four (a request's reads then run in several launches at
+    once and may finish out of order), every request equals its reads searched alone."""
+    print(f"case server mix: 9 requests, 2 option sets, {slots} slots", flush=True)
     w, reqs = requests(golden_dir)
     meta, words = ix.read_bwt(w["bwt"])
     exp = w["expected"]
@@ -56,7 +59,7 @@ def test_requests_in_a_mix_equal_alone_and_reference(golden_dir):
             rs.search(nv.FORM_BATCHED, nv.GALN_HIP_TRACE)
             alone.append(rs.results(want_pops=True, want_trace=True))
             rs.close()
-        with nv.Server(idx, max_launch_reads=512, quantum_reads=96, flags=nv.GALN_HIP_TRACE) as srv:
+        with nv.Server(idx, max_launch_reads=512, quantum_reads=96, flags=nv.GALN_HIP_TRACE, slots=slots) as srv:
             ids = [srv.submit(*arrays) for _, _, _, arrays in reqs]
             assert all(i is not None for i in ids)
             got = []
@@ -66,6 +69,7 @@ def test_requests_in_a_mix_equal_alone_and_reference(golden_dir):
             st = srv.stats()
         assert st["requests"] == len(reqs) and st["reads"] == sum(hi - lo for _, lo, hi, _ in reqs)
         assert st["launches"] >= st["reads"] // 512
+        assert st["slots"] == slots and 1 <= st["most_in_flight"] <= slots
     for (key, lo, hi, _), a, (g, lat) in zip(reqs, alone, got):
         assert lat >= 0.0
         np.testing.assert_array_equal(g.n_aln, a.n_aln)
