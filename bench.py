@@ -286,6 +286,44 @@ def serving_sweep(w, index, nv, loads, seconds: float, max_req: int, seed: int =
     return out
 
 
+def end_to_end(w, workdir: str) -> dict:
+    """`galn aln` on the workload's reads as a gzip FASTQ, from the command line to the .sai on disk:
+    streamed (blocks of four bwa batches read, searched and written at the same time) and with the
+    whole input read first; both .sai files must be the same bytes."""
+    import gzip
+    from galn_amd import native as nv
+    fq = os.path.join(workdir, "reads.fq.gz")
+    t = time.time()
+    step = 1 << 20
+    with gzip.open(fq, "wb", compresslevel=1) as fh:
+        for s in range(0, w.n, step):
+            e = min(w.n, s + step)
+            L = w.lens[s:e]
+            rows = w.reads[s:e]
+            fh.write(b"".join(b"@r%d\n%s\n+\n%s\n" % (s + i, rows[i, :L[i]].tobytes(), b"I" * int(L[i]))
+                              for i in range(e - s)))
+    log(f"end to end: wrote {w.n} reads as {os.path.getsize(fq) / 1e6:.1f} MB of gzip FASTQ in {time.time() - t:.1f} s")
+    res = {"reads": w.n, "fq_gz_bytes": os.path.getsize(fq)}
+    md5 = {}
+    for name, extra in (("streamed", []), ("whole_input", ["--block-batches", "0"])):
+        sai = os.path.join(workdir, f"e2e_{name}.sai")
+        t = time.perf_counter()
+        r = subprocess.run([nv.GALN_BIN, "aln", *extra, *w.opts, "-f", sai, w.prefix, fq], capture_output=True, text=True)
+        el = time.perf_counter() - t
+        if r.returncode != 0:
+            raise RuntimeError(f"galn aln failed ({r.returncode}): {r.stderr}")
+        import hashlib
+        h = hashlib.md5()
+        with open(sai, "rb") as fh:
+            for b in iter(lambda: fh.read(1 << 24), b""):
+                h.update(b)
+        md5[name] = h.hexdigest()
+        res[name] = {"seconds": el, "reads_per_s": w.n / el, "galn_stderr": r.stderr.strip().splitlines()[-1]}
+        log(f"end to end, {name}: {el:.2f} s ({w.n / el:.3e} reads/s): {res[name]['galn_stderr']}")
+    res["sai_equal"] = md5["streamed"] == md5["whole_input"]
+    return res
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -310,6 +348,8 @@ def main() -> int:
                          "kernel that scales the FETCH_SIZE / WRITE_SIZE counters)")
     ap.add_argument("--autotune-reads", type=int, default=0,
                     help="reads the autotuner searches (the first ones; 0: all for adna/plain, 200k otherwise)")
+    ap.add_argument("--end-to-end", action="store_true",
+                    help="also time `galn aln` on the reads as a gzip FASTQ, streamed and whole-input")
     ap.add_argument("--workdir", default="")
     args = ap.parse_args()
     # the library reports autotune trials and index-build chunks on stderr (long, otherwise silent calls)
@@ -535,6 +575,8 @@ def main() -> int:
         out["parity"]["cpu_sample_eq_gpu"] = cb["records_equal_gpu"]
         out["parity"]["cpu_sample_reads"] = sample
         log(f"CPU baseline: {cb['value']:.3e} reads/s ({cb['kind']}), records equal: {cb['records_equal_gpu']}")
+    if args.end_to_end:
+        out["end_to_end"] = end_to_end(w, workdir)
     if not args.workdir:
         shutil.rmtree(workdir, ignore_errors=True)
     print(json.dumps(out), flush=True)

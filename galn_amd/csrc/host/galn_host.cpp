@@ -10,6 +10,10 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <condition_variable>
+#include <deque>
+#include <memory>
+#include <mutex>
 #include <string>
 #include <thread>
 #include <vector>
@@ -62,6 +66,7 @@ extern "C" int galn_parse_aln_args(int argc, char **argv, galn_aln_args_t *a)
     galn_gap_opt_default(&a->opt);
     galn_gap_opt_t *o = &a->opt;
     a->form = GALN_FORM_BATCHED;
+    a->block_batches = 4;
     for (int i = 0; i < argc; ++i) {
         const char *s = argv[i];
         const char *v = (i + 1 < argc) ? argv[i + 1] : nullptr;
@@ -115,6 +120,13 @@ extern "C" int galn_parse_aln_args(int argc, char **argv, galn_aln_args_t *a)
         else if (!strcmp(s, "--device")) a->device = atoi(v);
         else if (!strcmp(s, "--cpu-workers")) a->cpu_threads = atoi(v);
         else if (!strcmp(s, "--cpu-interleave")) a->cpu_interleave = atoi(v);
+        else if (!strcmp(s, "--block-batches")) {
+            a->block_batches = atoi(v);
+            if (a->block_batches < 0 || a->block_batches > 64) {
+                fprintf(stderr, "[galn] --block-batches takes 0 (whole input) to 64\n");
+                return -1;
+            }
+        }
         else if (!strcmp(s, "--form")) {
             if (!strcmp(v, "plain")) a->form = GALN_FORM_PLAIN;
             else if (!strcmp(v, "batched")) a->form = GALN_FORM_BATCHED;
@@ -286,27 +298,112 @@ extern "C" void galn_free_bwt(uint32_t *bwt) { free(bwt); }
 
 namespace {
 
+// gzip (or plain) input: a thread inflates ahead of the parser into a ring of chunks, as the
+// reference's inflate thread does (src/host_io.c:79-130), so that decompression and parsing
+// overlap
+class InflateRing {
+  public:
+    static constexpr int NCH = 16;
+    static constexpr int CH = 4 << 20;
+    ~InflateRing() { close(); }
+    bool open(const char *fn)
+    {
+        fp_ = gzopen(fn, "rb");
+        if (!fp_) return false;
+        gzbuffer(fp_, 1 << 20);
+        buf_.assign(NCH, std::vector<unsigned char>(CH));
+        len_.assign(NCH, 0);
+        th_ = std::thread([this] { run(); });
+        return true;
+    }
+    // the next chunk, valid until the next call (which hands it back); 0 at the end of input
+    int next(const unsigned char **p)
+    {
+        std::unique_lock<std::mutex> lk(mu_);
+        if (holding_) {
+            ++tail_;
+            holding_ = false;
+            cv_.notify_all();
+        }
+        if (eof_) return 0;
+        cv_.wait(lk, [&] { return head_ > tail_; });
+        const int k = (int)(tail_ % NCH);
+        if (len_[k] == 0) {
+            eof_ = true;
+            return 0;
+        }
+        holding_ = true;
+        *p = buf_[k].data();
+        return len_[k];
+    }
+    // stops the thread; true when everything read so far decompressed without error
+    bool close()
+    {
+        if (!fp_) return closed_ok_;
+        {
+            std::lock_guard<std::mutex> lk(mu_);
+            stop_ = true;
+        }
+        cv_.notify_all();
+        if (th_.joinable()) th_.join();
+        const int zc = gzclose(fp_);
+        fp_ = nullptr;
+        closed_ok_ = !err_ && zc == Z_OK;
+        return closed_ok_;
+    }
+    bool decompression_error() const { return err_; }
+
+  private:
+    void run()
+    {
+        for (;;) {
+            int k;
+            {
+                std::unique_lock<std::mutex> lk(mu_);
+                cv_.wait(lk, [&] { return stop_ || head_ - tail_ < NCH; });
+                if (stop_) return;
+                k = (int)(head_ % NCH);
+            }
+            int r = gzread(fp_, buf_[k].data(), (unsigned)CH);
+            std::lock_guard<std::mutex> lk(mu_);
+            if (r < 0) {
+                err_ = true;  // a decompression error is not taken for the end of input
+                r = 0;
+            }
+            len_[k] = r;
+            ++head_;
+            cv_.notify_all();
+            if (r == 0) return;
+        }
+    }
+    gzFile fp_ = nullptr;
+    std::vector<std::vector<unsigned char>> buf_;
+    std::vector<int> len_;
+    long head_ = 0, tail_ = 0;
+    bool holding_ = false, eof_ = false, stop_ = false, err_ = false, closed_ok_ = false;
+    std::mutex mu_;
+    std::condition_variable cv_;
+    std::thread th_;
+};
+
 struct GzStream {
-    gzFile fp = nullptr;
-    std::vector<unsigned char> buf;
+    InflateRing *ring;
+    const unsigned char *data = nullptr;
     int begin = 0, end = 0;
     bool eof = false;
-    bool error = false;
-    explicit GzStream(gzFile f) : fp(f), buf(1 << 20) {}
+    explicit GzStream(InflateRing *r) : ring(r) {}
     bool fill()
     {
         if (eof) return false;
-        const int r = gzread(fp, buf.data(), (unsigned)buf.size());
-        if (r < 0) error = true;
         begin = 0;
-        end = r > 0 ? r : 0;
+        end = ring->next(&data);
         if (end == 0) eof = true;
         return end > 0;
     }
     int getc()
     {
         if (begin >= end && !fill()) return -1;
-        return buf[begin++];
+        return data[begin++];
     }
     // append bytes up to (not including) the delimiter; the delimiter is consumed.
     // mode 0: whitespace, 1: newline. Returns -1 if nothing was read at the end of input,
@@ -320,15 +417,16 @@ struct GzStream {
             if (begin >= end && !fill()) break;
             int i = begin;
             if (mode == 1) {
-                while (i < end && buf[i] != '\n') ++i;
+                const void *nl = memchr(data + begin, '\n', (size_t)(end - begin));
+                i = nl ? (int)((const unsigned char *)nl - data) : end;
             } else {
-                while (i < end && !isspace(buf[i])) ++i;
+                while (i < end && !isspace(data[i])) ++i;
             }
             gotany = true;
-            s.append((const char *)buf.data() + begin, (size_t)(i - begin));
+            s.append((const char *)data + begin, (size_t)(i - begin));
             begin = i + 1;
             if (i < end) {
-                if (dret) *dret = buf[i];
+                if (dret) *dret = data[i];
                 break;
             }
         }
@@ -400,14 +498,13 @@ struct galn_readbuf {
 
 extern "C" galn_readbuf_t *galn_read_fastq(const char *fn)
 {
-    // src/host_io.c:188-224 (bwa_read_seq batches)
-    gzFile fp = gzopen(fn, "rb");
-    if (!fp) {
+    // src/host_io.c:151-205 (bwa_read_seq batches)
+    InflateRing ring;
+    if (!ring.open(fn)) {
         fprintf(stderr, "[galn] cannot open %s\n", fn);
         return nullptr;
     }
-    gzbuffer(fp, 1 << 20);
-    GzStream ks(fp);
+    GzStream ks(&ring);
     Record rec;
     galn_readbuf *rb = new galn_readbuf;
     int batch = 0, in_batch = 0;
@@ -438,9 +535,9 @@ extern "C" galn_readbuf_t *galn_read_fastq(const char *fn)
             in_batch = 0;
         }
     }
-    const int zc = gzclose(fp);
-    if (ks.error || zc != Z_OK) {
-        fprintf(stderr, "[galn] error reading %s (%s)\n", fn, ks.error ? "decompression error" : "truncated or corrupt input");
+    if (!ring.close()) {
+        fprintf(stderr, "[galn] error reading %s (%s)\n", fn,
+                ring.decompression_error() ? "decompression error" : "truncated or corrupt input");
         delete rb;
         return nullptr;
     }
@@ -460,11 +557,141 @@ extern "C" const char *galn_readbuf_name(const galn_readbuf_t *r, int32_t i)
 }
 extern "C" void galn_readbuf_free(galn_readbuf_t *r) { delete r; }
 
+namespace {
+
+// A block of whole bwa batches of the input, for the streamed path: queries, lengths, batch ids
+// counted from 0 within the block, names.
+struct ReadBlock {
+    std::vector<uint8_t> query;
+    std::vector<uint64_t> qoff;
+    std::vector<int32_t> qlen, batch;
+    std::vector<uint64_t> name_off;
+    std::vector<char> names;
+    int n_batches = 0;
+    int n() const { return (int)qlen.size(); }
+    const char *name(int i) const { return names.data() + name_off[(size_t)i]; }
+};
+
+// Parses the input on a thread of its own into blocks of `max_batches` bwa batches, up to `ahead`
+// blocks ahead of the consumer, with the record and batch rules of galn_read_fastq.
+class BlockReader {
+  public:
+    ~BlockReader() { stop(); }
+    bool open(const char *fn, int max_batches, int ahead)
+    {
+        if (!ring_.open(fn)) return false;
+        max_batches_ = max_batches;
+        ahead_ = ahead;
+        th_ = std::thread([this] { run(); });
+        return true;
+    }
+    // the next block in input order; false at the end of input (see ok())
+    bool next(std::unique_ptr<ReadBlock> &b)
+    {
+        std::unique_lock<std::mutex> lk(mu_);
+        cv_.wait(lk, [&] { return !q_.empty() || done_; });
+        if (q_.empty()) return false;
+        b = std::move(q_.front());
+        q_.pop_front();
+        cv_.notify_all();
+        return true;
+    }
+    // after the end: the input decompressed without error
+    bool ok()
+    {
+        std::unique_lock<std::mutex> lk(mu_);
+        cv_.wait(lk, [&] { return done_; });
+        return ok_;
+    }
+    bool decompression_error() const { return ring_.decompression_error(); }
+    void stop()
+    {
+        {
+            std::lock_guard<std::mutex> lk(mu_);
+            stop_ = true;
+        }
+        cv_.notify_all();
+        if (th_.joinable()) th_.join();
+    }
+
+  private:
+    void push(std::unique_ptr<ReadBlock> &b, int n_batches)
+    {
+        b->n_batches = n_batches;
+        std::unique_lock<std::mutex> lk(mu_);
+        cv_.wait(lk, [&] { return (int)q_.size() < ahead_ || stop_; });
+        q_.push_back(std::move(b));
+        cv_.notify_all();
+        b.reset(new ReadBlock);
+    }
+    void run()
+    {
+        GzStream ks(&ring_);
+        Record rec;
+        auto blk = std::make_unique<ReadBlock>();
+        int local = 0, in_batch = 0;  // batches completed in this block; reads in the current batch
+        for (;;) {
+            {
+                std::lock_guard<std::mutex> lk(mu_);
+                if (stop_) break;
+            }
+            const int l = read_record(ks, rec);
+            if (l < 0) {
+                // a malformed record ends the batch; an empty batch ends the input (src/host_io.c:164-186)
+                if (in_batch == 0 || l == -1) break;
+                in_batch = 0;
+                if (++local == max_batches_) {
+                    push(blk, local);
+                    local = 0;
+                }
+                continue;
+            }
+            if (rec.seq.empty()) continue;
+            const int len = (int)rec.seq.size();
+            blk->qoff.push_back(blk->query.size());
+            blk->qlen.push_back(len);
+            blk->batch.push_back(local);
+            const size_t o = blk->query.size();
+            blk->query.resize(o + (size_t)len);
+            for (int j = 0; j < len; ++j) {
+                const unsigned char b = nt4_of((unsigned char)rec.seq[(size_t)(len - 1 - j)]);
+                blk->query[o + (size_t)j] = b > 3 ? 4 : (uint8_t)(3 - b);
+            }
+            blk->name_off.push_back(blk->names.size());
+            blk->names.insert(blk->names.end(), rec.name.begin(), rec.name.end());
+            blk->names.push_back('\0');
+            if (++in_batch == GALN_HOST_ALN_BATCH) {
+                in_batch = 0;
+                if (++local == max_batches_) {
+                    push(blk, local);
+                    local = 0;
+                }
+            }
+        }
+        if (blk->n() > 0) push(blk, local + (in_batch > 0 ? 1 : 0));
+        const bool ok = ring_.close();
+        std::lock_guard<std::mutex> lk(mu_);
+        ok_ = ok;
+        done_ = true;
+        cv_.notify_all();
+    }
+    InflateRing ring_;
+    int max_batches_ = 4, ahead_ = 3;
+    std::deque<std::unique_ptr<ReadBlock>> q_;
+    bool done_ = false, ok_ = false, stop_ = false;
+    std::mutex mu_;
+    std::condition_variable cv_;
+    std::thread th_;
+};
+
+}  // namespace
+
 // ---------------------------------------------------------------------------
 // .sai
 
-extern "C" int galn_sai_write(FILE *fp, const galn_gap_opt_t *o, int32_t n, const int32_t *n_aln,
-                              const uint64_t *rec_off, const galn_aln_rec_t *rec)
+// per read in input order: int32 n_aln (an errored read: 0) and its records
+static int sai_write_reads(FILE *fp, const galn_gap_opt_t *o, int32_t n, const int32_t *n_aln, const uint64_t *rec_off,
+                           const galn_aln_rec_t *rec, bool header)
 {
     // src/sai_writer.h:9-21, src/galn.cu:1250-1263
     static_assert(sizeof(galn_gap_opt_t) == 64, "gap_opt_t is 64 bytes");
@@ -479,7 +706,7 @@ extern "C" int galn_sai_write(FILE *fp, const galn_gap_opt_t *o, int32_t n, cons
         }
         return true;
     };
-    bool ok = put("SAI\1", 4) && put(o, sizeof(*o));
+    bool ok = !header || (put("SAI\1", 4) && put(o, sizeof(*o)));
     for (int32_t i = 0; i < n && ok; ++i) {
         const int32_t na = n_aln[i] > 0 ? n_aln[i] : 0;
         ok = put(&na, 4);
@@ -488,6 +715,12 @@ extern "C" int galn_sai_write(FILE *fp, const galn_gap_opt_t *o, int32_t n, cons
     if (ok && !buf.empty()) ok = fwrite(buf.data(), 1, buf.size(), fp) == buf.size();
     if (!ok || fflush(fp) != 0 || ferror(fp)) return 5;
     return 0;
+}
+
+extern "C" int galn_sai_write(FILE *fp, const galn_gap_opt_t *o, int32_t n, const int32_t *n_aln,
+                              const uint64_t *rec_off, const galn_aln_rec_t *rec)
+{
+    return sai_write_reads(fp, o, n, n_aln, rec_off, rec, true);
 }
 
 // ---------------------------------------------------------------------------
@@ -532,6 +765,9 @@ static int usage()
             "         --cpu-workers INT  CPU search threads [all hardware threads]\n"
             "         --cpu-interleave INT  reads one CPU thread advances in turn [8]\n"
             "         --verify           re-run every read on the CPU and compare (exit 6 on a difference)\n"
+            "         --block-batches INT  stream the input to the GPU in blocks of INT bwa batches of\n"
+            "                            0x40000 reads, read, searched and written at the same time;\n"
+            "                            0 reads the whole input first [4]\n"
             "         -v                 print timing to stderr\n\n");
     return 1;
 }
@@ -582,6 +818,189 @@ static int cpu_run(const uint32_t *bwt, const galn_bwt_meta_t *meta, const galn_
     }
 }
 
+// The reads a GPU search left as outside its kernels' limits (GALN_READ_ERR_UNSUP), searched on the CPU
+// engine and merged into `res` in read order, so the records are the same whichever engine found them.
+static int finish_unsupported(const uint32_t *bwt, const galn_bwt_meta_t *meta, const galn_hip_reads *R,
+                              const galn_aln_args_t &A, SearchOut &res, int *n_cpu)
+{
+    const int n = R->n_reads;
+    std::vector<int32_t> cpu_list;
+    for (int i = 0; i < n; ++i)
+        if (res.n_aln[(size_t)i] == GALN_READ_ERR_UNSUP) cpu_list.push_back(i);
+    *n_cpu = (int)cpu_list.size();
+    if (cpu_list.empty()) return 0;
+    SearchOut c;
+    const int r = cpu_run(bwt, meta, R, &cpu_list, A, c);
+    if (r) return r;
+    std::vector<galn_aln_rec_t> rec;
+    rec.reserve(res.rec.size() + c.rec.size());
+    size_t j = 0;
+    for (int i = 0; i < n; ++i) {
+        const bool on_cpu = j < cpu_list.size() && cpu_list[j] == i;
+        const galn_aln_rec_t *src = on_cpu ? c.rec.data() + c.rec_off[j] : res.rec.data() + res.rec_off[(size_t)i];
+        if (on_cpu) {
+            res.n_aln[(size_t)i] = c.n_aln[j];
+            res.pops[(size_t)i] = c.pops[j];
+            res.trace[(size_t)i] = c.trace[j];
+            ++j;
+        }
+        res.rec_off[(size_t)i] = rec.size();
+        if (res.n_aln[(size_t)i] > 0) rec.insert(rec.end(), src, src + res.n_aln[(size_t)i]);
+    }
+    res.rec.swap(rec);
+    return 0;
+}
+
+// The streamed GPU path: blocks of whole bwa batches are read and parsed on their own threads,
+// searched by the serving loop (up to three launches on the device at once, so one block's
+// search overlaps the next) and written in input order as they finish. Returns the exit code.
+static int aln_stream(const galn_aln_args_t &A, FILE *out, const uint32_t *bwt, const galn_bwt_meta_t &meta,
+                      BlockReader &br, double t0, double t_loaded)
+{
+    galn_hip_index *idx = nullptr;
+    if (galn_hip_index_open(bwt, &meta, A.device, &idx)) {
+        fprintf(stderr, "[galn] GPU: %s (use --cpu-only to run without a GPU)\n", galn_hip_last_error());
+        return 10;
+    }
+    const bool want_trace = A.trace != 0;
+    galn_hip_server_cfg_t cfg{};
+    cfg.max_launch_reads = A.block_batches * GALN_HOST_ALN_BATCH;
+    cfg.quantum_reads = cfg.max_launch_reads;
+    cfg.max_queued_bytes = (uint64_t)64 << 30;
+    cfg.flags = want_trace ? GALN_HIP_TRACE : 0u;
+    cfg.form = A.form;
+    cfg.slots = 3;
+    galn_hip_server *S = nullptr;
+    if (galn_hip_server_open(idx, &cfg, &S)) {
+        fprintf(stderr, "[galn] GPU: %s\n", galn_hip_last_error());
+        galn_hip_index_close(idx);
+        return 10;
+    }
+    struct Pend {
+        std::unique_ptr<ReadBlock> b;
+        std::vector<galn_search_opt_t> bopts;
+        std::vector<int32_t> nstacks, maxdiff;
+        galn_hip_reads R{};
+        uint64_t id = 0;
+    };
+    std::deque<Pend> pend;
+    int err = 0, fatal = 0, n_cpu = 0;
+    long n_reads = 0, mapped = 0, n_rec = 0, blocks = 0;
+    double t_cpu = 0, pops_total = 0;
+    if (sai_write_reads(out, &A.opt, 0, nullptr, nullptr, nullptr, true)) err = 5;
+    // the oldest block's results, completed on the CPU where needed, onto the .sai
+    auto collect_front = [&]() {
+        Pend &p = pend.front();
+        const int n = p.b->n();
+        int r = galn_hip_server_wait(S, p.id, -1);
+        SearchOut res;
+        res.resize(n);
+        galn_hip_out O = res.view(want_trace);
+        double lat = 0;
+        if (r >= 0) {
+            r = galn_hip_server_result(S, p.id, &O, &lat);
+            if (r == GALN_HIP_E_RECCAP) {
+                res.rec.resize(O.n_rec);
+                O = res.view(want_trace);
+                r = galn_hip_server_result(S, p.id, &O, &lat);
+            }
+        }
+        if (r < 0) {
+            fprintf(stderr, "[galn] GPU search failed: %s\n", galn_hip_last_error());
+            fatal = 10;
+        } else {
+            res.rec.resize(O.n_rec);
+            const double tc = now_s();
+            int nc = 0;
+            if ((r = finish_unsupported(bwt, &meta, &p.R, A, res, &nc))) {
+                fprintf(stderr, "[galn] CPU search failed (%d)\n", r);
+                fatal = 10;
+            }
+            n_cpu += nc;
+            if (nc) t_cpu += now_s() - tc;
+            for (int i = 0; i < n && !fatal; ++i) {
+                if (res.n_aln[(size_t)i] < 0) {
+                    fprintf(stderr, "[galn] read %ld (%s): search error %d\n", n_reads + i, p.b->name(i),
+                            res.n_aln[(size_t)i]);
+                    err = err ? err : 4;
+                } else if (res.n_aln[(size_t)i] > 0) {
+                    ++mapped;
+                    n_rec += res.n_aln[(size_t)i];
+                }
+                pops_total += (double)res.pops[(size_t)i];
+            }
+            // a read with an error is written with 0 records (exit 4); after a write error, nothing more
+            if (!fatal && err != 5 &&
+                sai_write_reads(out, &A.opt, n, res.n_aln.data(), res.rec_off.data(), res.rec.data(), false))
+                err = 5;
+        }
+        n_reads += n;
+        ++blocks;
+        pend.pop_front();
+    };
+    std::unique_ptr<ReadBlock> b;
+    while (!fatal && br.next(b)) {
+        Pend p;
+        p.b = std::move(b);
+        const int n = p.b->n(), nb = std::max(1, p.b->n_batches);
+        p.bopts.resize((size_t)nb);
+        p.nstacks.resize((size_t)nb);
+        p.maxdiff.resize((size_t)std::max(n, 1));
+        galn_prepare_batches(&A.opt, n, p.b->qlen.data(), p.b->batch.data(), nb, p.bopts.data(), p.nstacks.data(),
+                             p.maxdiff.data());
+        p.R.n_reads = n;
+        p.R.query = p.b->query.data();
+        p.R.qoff = p.b->qoff.data();
+        p.R.qlen = p.b->qlen.data();
+        p.R.max_diff = p.maxdiff.data();
+        p.R.batch = p.b->batch.data();
+        p.R.n_batches = nb;
+        p.R.bopts = p.bopts.data();
+        p.R.n_stacks = p.nstacks.data();
+        for (;;) {
+            const int r = galn_hip_server_submit(S, &p.R, &p.id);
+            if (r == GALN_HIP_E_BUSY && !pend.empty()) {
+                collect_front();
+                continue;
+            }
+            if (r) {
+                fprintf(stderr, "[galn] GPU: %s\n", galn_hip_last_error());
+                fatal = 10;
+            }
+            break;
+        }
+        if (fatal) break;
+        pend.push_back(std::move(p));
+        // keep the blocks in flight bounded; write the oldest as soon as it is done
+        while (!fatal && !pend.empty() && ((int)pend.size() > 4 || galn_hip_server_wait(S, pend.front().id, 0) != 0))
+            collect_front();
+    }
+    while (!fatal && !pend.empty()) collect_front();
+    const bool input_ok = fatal ? true : br.ok();
+    br.stop();
+    galn_hip_server_close(S);
+    galn_hip_index_close(idx);
+    if (out != stdout && fclose(out) != 0 && !err) err = 5;
+    if (fatal) return fatal;
+    if (!input_ok) {
+        fprintf(stderr, "[galn] error reading %s (%s)\n", A.reads,
+                br.decompression_error() ? "decompression error" : "truncated or corrupt input");
+        return 3;
+    }
+    const double t_end = now_s();
+    fprintf(stderr, "[galn] %ld reads (%ld with hits, %ld records) in %.2f s: index load %.2f s, streamed read + search + "
+                    "write %.2f s (%ld blocks)\n",
+            n_reads, mapped, n_rec, t_end - t0, t_loaded - t0, t_end - t_loaded, blocks);
+    if (A.verbose) {
+        fprintf(stderr, "[galn] GPU %d, %s form, streamed: %.3e pops, %.3e reads/s end to end\n", A.device,
+                A.form == GALN_FORM_PLAIN ? "plain" : "batched", pops_total, n_reads / std::max(t_end - t0, 1e-9));
+        if (n_cpu)
+            fprintf(stderr, "[galn] CPU engine: %d reads in %.3f s (%.3e reads/s)\n", n_cpu, t_cpu,
+                    t_cpu > 0 ? n_cpu / t_cpu : 0.0);
+    }
+    return err;
+}
+
 extern "C" int galn_aln_main(int argc, char **argv)
 {
     galn_aln_args_t A;
@@ -593,6 +1012,25 @@ extern "C" int galn_aln_main(int argc, char **argv)
         return 1;
     }
     std::string bwtfn = std::string(A.prefix) + ".bwt";
+    if (!A.cpu_only && !A.verify && A.block_batches > 0) {
+        // the streamed GPU path: the input is parsed while the index loads
+        BlockReader br;
+        const bool opened = br.open(A.reads, A.block_batches, 3);
+        galn_bwt_meta_t meta;
+        uint32_t *bwt = galn_load_bwt(bwtfn.c_str(), &meta);
+        if (!bwt) {
+            br.stop();
+            return 2;
+        }
+        if (!opened) {
+            fprintf(stderr, "[galn] cannot open %s\n", A.reads);
+            galn_free_bwt(bwt);
+            return 3;
+        }
+        const int rc = aln_stream(A, out, bwt, meta, br, t0, now_s());
+        galn_free_bwt(bwt);
+        return rc;
+    }
     galn_readbuf_t *reads = nullptr;
     std::thread fq([&] { reads = galn_read_fastq(A.reads); });
     galn_bwt_meta_t meta;
@@ -679,38 +1117,14 @@ extern "C" int galn_aln_main(int argc, char **argv)
         res.rec.resize(O.n_rec);
         // the reads outside the kernels' limits finish on the CPU engine, as the reference hands
         // them to its CPU workers (src/gapw.cuh:753-755, src/galn.cu:1151-1180)
-        std::vector<int32_t> cpu_list;
-        for (int i = 0; i < n; ++i)
-            if (res.n_aln[(size_t)i] == GALN_READ_ERR_UNSUP) cpu_list.push_back(i);
-        if (!cpu_list.empty()) {
-            const double tc = now_s();
-            SearchOut c;
-            r = cpu_run(bwt, &meta, &R, &cpu_list, A, c);
-            if (r) {
-                fprintf(stderr, "[galn] CPU search failed (%d)\n", r);
-                cleanup();
-                return 10;
-            }
-            // merge: records of every read in read order, from whichever engine searched it
-            std::vector<galn_aln_rec_t> rec;
-            rec.reserve(res.rec.size() + c.rec.size());
-            size_t j = 0;
-            for (int i = 0; i < n; ++i) {
-                const bool on_cpu = j < cpu_list.size() && cpu_list[j] == i;
-                const galn_aln_rec_t *src = on_cpu ? c.rec.data() + c.rec_off[j] : res.rec.data() + res.rec_off[(size_t)i];
-                if (on_cpu) {
-                    res.n_aln[(size_t)i] = c.n_aln[j];
-                    res.pops[(size_t)i] = c.pops[j];
-                    res.trace[(size_t)i] = c.trace[j];
-                    ++j;
-                }
-                res.rec_off[(size_t)i] = rec.size();
-                if (res.n_aln[(size_t)i] > 0) rec.insert(rec.end(), src, src + res.n_aln[(size_t)i]);
-            }
-            res.rec.swap(rec);
-            n_cpu = (int)cpu_list.size();
-            t_cpu = now_s() - tc;
+        const double tc = now_s();
+        r = finish_unsupported(bwt, &meta, &R, A, res, &n_cpu);
+        if (r) {
+            fprintf(stderr, "[galn] CPU search failed (%d)\n", r);
+            cleanup();
+            return 10;
         }
+        if (n_cpu) t_cpu = now_s() - tc;
     }
     const double t_search = now_s();
 

@@ -93,7 +93,8 @@ class GapOpt(C.Structure):
 class AlnArgs(C.Structure):
     _fields_ = [("opt", GapOpt), ("prefix", C.c_char_p), ("reads", C.c_char_p), ("out", C.c_char_p),
                 ("device", C.c_int), ("form", C.c_int), ("verbose", C.c_int), ("trace", C.c_int),
-                ("cpu_only", C.c_int), ("verify", C.c_int), ("cpu_threads", C.c_int), ("cpu_interleave", C.c_int)]
+                ("cpu_only", C.c_int), ("verify", C.c_int), ("cpu_threads", C.c_int), ("cpu_interleave", C.c_int),
+                ("block_batches", C.c_int)]
 
 
 class GalnError(RuntimeError):

@@ -61,6 +61,9 @@ typedef struct {
                                records, pops and traces (exit 6 on a difference) */
     int cpu_threads;        /* --cpu-workers N: CPU engine threads [all hardware threads] */
     int cpu_interleave;     /* --cpu-interleave N: reads a CPU thread advances in turn [8] */
+    int block_batches;      /* --block-batches N: the GPU path streams the input in blocks of N bwa
+                               batches (0x40000 reads each), searched while the next are read and
+                               the last written [4]; 0: read the whole input first */
 } galn_aln_args_t;
 
 void galn_gap_opt_default(galn_gap_opt_t *o);

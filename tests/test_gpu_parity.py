@@ -114,6 +114,42 @@ def test_cli_writes_golden_sai(name, key, form, golden_dir, tmp_path):
     assert fx.md5_file(str(out)) == w["meta"]["runs"][key]["sai_md5"]
 
 
+@pytest.mark.parametrize("key", ["adna_1", "default_1"])
+def test_cli_streams_blocks_of_one_batch(key, golden_dir, tmp_path):
+    """The edge FASTQ has two bwa batches (a malformed record ends the first): streamed one batch
+    per block, the .sai is still the reference's."""
+    print(f"case cli stream fastq_edge/{key}", flush=True)
+    w = golden_inputs("fastq_edge", golden_dir)
+    opts, mate = run_opts(w["meta"], key)
+    out = tmp_path / "out.sai"
+    r = subprocess.run([nv.GALN_BIN, "aln", "--block-batches", "1", *opts, "-f", str(out), w["prefix"], w["reads"][mate]],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    assert "(2 blocks)" in r.stderr
+    assert fx.md5_file(str(out)) == w["meta"]["runs"][key]["sai_md5"]
+
+
+def test_cli_streamed_equals_whole_input_on_many_batches(golden_dir, tmp_path):
+    """300k reads (two bwa batches, gzip input): streamed in blocks of one batch, of four, and
+    read whole first, the .sai bytes are the same."""
+    import gzip
+    print("case cli stream 300k reads", flush=True)
+    w = golden_inputs("adna_mixed", golden_dir)
+    src = open(w["reads"][0], "rb").read()
+    fq = tmp_path / "many.fq.gz"
+    with gzip.open(fq, "wb", compresslevel=1) as fh:
+        for _ in range(38):  # 38 x 8000 reads
+            fh.write(src)
+    outs = {}
+    for bb in ("1", "4", "0"):
+        out = tmp_path / f"out{bb}.sai"
+        r = subprocess.run([nv.GALN_BIN, "aln", "--block-batches", bb, "-l", "1024", "-n", "0.01", "-o", "2", "-f",
+                            str(out), w["prefix"], str(fq)], capture_output=True, text=True, timeout=300)
+        assert r.returncode == 0, r.stderr
+        outs[bb] = fx.md5_file(str(out))
+    assert outs["1"] == outs["4"] == outs["0"]
+
+
 def test_algorithmic_bytes_match_oracle(golden_dir):
     """The kernel's per-read algorithmic byte count is the oracle's (same definition)."""
     w, o, rs = readset_for("adna_mixed", "adna_1", golden_dir)
