@@ -208,6 +208,8 @@ def serving_sweep(w, index, nv, loads, seconds: float, max_req: int, seed: int =
                 _, lat = srv.result(ids[k])
                 lats.append(lat)
                 k += 1
+                if k % 8 == 0 and k < len(reqs):
+                    log(f"  serving: {k} of {len(reqs)} requests done")
             done.set()
 
         th = threading.Thread(target=collect, daemon=True)
@@ -274,7 +276,11 @@ def serving_sweep(w, index, nv, loads, seconds: float, max_req: int, seed: int =
         b0 += len(bo)
     rs_mix = nv.ReadSet(index, np.concatenate(q_all), np.concatenate(qoff_all), np.concatenate(qls),
                         np.concatenate(mds), np.concatenate(batch_all).astype(np.int32), bopts_all, nst_all)
-    rs_mix.search(nv.FORM_BATCHED)
+    # one untimed search when a search of these requests is short (first-use costs); on a slow
+    # workload one timed search, whose first-use costs are a few ms of it
+    log(f"serving: the batched form on the same {len(peak_reqs)} requests ({n_reads} reads)")
+    if n_reads / peak < 20:
+        rs_mix.search(nv.FORM_BATCHED)
     t0 = time.perf_counter()
     rs_mix.search(nv.FORM_BATCHED)
     rs_mix.results()
