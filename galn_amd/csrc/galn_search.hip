@@ -2325,12 +2325,13 @@ struct galn_hip_readset {
     std::vector<galn_search_opt_t> h_bopts;
     // widths handed from k_search to k_heavy, one slot per heavy-list position
     uint64_t *d_hand = nullptr;
+    size_t hand_bytes = 0;
     uint32_t hand_stride = 0;
     int32_t hand_cap = 0;
     // results in read order: inclusive prefix sums of record counts, output offsets, staging
     uint64_t *d_incl = nullptr, *d_roff = nullptr;
     galn_aln_rec_t *d_stage = nullptr;
-    uint64_t stage_cap = 0;
+    size_t stage_bytes = 0;
     void *d_scan_tmp = nullptr;
     size_t scan_tmp_bytes = 0;
     // capacities of the device arrays (reads, query bytes, batches)
@@ -2388,15 +2389,31 @@ Layout make_layout(int max_len, int seed_len, int rec_cap, int own_cap)
     return L;
 }
 
+// A device buffer of at least `need` bytes (contents not kept). It grows to twice its size or
+// more: hipFree waits for every stream of the device, so a read set whose launches vary in size
+// (a server slot) reallocates a few times rather than every launch.
+int grow_dev(void **p, size_t &cap, size_t need, hipStream_t s)
+{
+    if (*p && cap >= need) return 0;
+    HIP_OK(hipStreamSynchronize(s));
+    if (*p) HIP_OK(hipFree(*p));
+    *p = nullptr;
+    const size_t c = std::max(need, cap * 2);
+    if (hipMalloc(p, c) != hipSuccess) {
+        (void)hipGetLastError();
+        *p = nullptr;
+        cap = 0;
+        if (hipMalloc(p, need) != hipSuccess) return fail(GALN_HIP_E_NOMEM, "device allocation of %zu bytes failed", need);
+        cap = need;
+        return 0;
+    }
+    cap = c;
+    return 0;
+}
+
 int ensure_scratch(galn_hip_readset *rs, size_t bytes)
 {
-    if (rs->scratch_bytes >= bytes) return 0;
-    if (rs->d_scratch) HIP_OK(hipFree(rs->d_scratch));
-    rs->d_scratch = nullptr;
-    rs->scratch_bytes = 0;
-    HIP_OK(hipMalloc(&rs->d_scratch, bytes));
-    rs->scratch_bytes = bytes;
-    return 0;
+    return grow_dev((void **)&rs->d_scratch, rs->scratch_bytes, bytes, rs->ctx->stream);
 }
 
 // Granules the stack pool may hold at most: no limit, or the value of GALN_HIP_TEST_POOL_GRANULES,
@@ -2601,14 +2618,7 @@ int prepare_heavy(galn_hip_readset *rs, int waves, int rec_cap)
     if (r) return r;
     const int own_cap = (int)std::min<uint64_t>(cx->n_gran, 32768);
     const HLayout L = make_hlayout(rec_cap, own_cap);
-    if (rs->hscratch_bytes < (size_t)waves * L.slot_bytes) {
-        if (rs->d_hscratch) HIP_OK(hipFree(rs->d_hscratch));
-        rs->d_hscratch = nullptr;
-        rs->hscratch_bytes = 0;
-        HIP_OK(hipMalloc(&rs->d_hscratch, (size_t)waves * L.slot_bytes));
-        rs->hscratch_bytes = (size_t)waves * L.slot_bytes;
-    }
-    return 0;
+    return grow_dev((void **)&rs->d_hscratch, rs->hscratch_bytes, (size_t)waves * L.slot_bytes, cx->stream);
 }
 
 // Hand-off slots for the widths of reads moving to the wave engine: one per read, up to
@@ -2620,17 +2630,20 @@ int prepare_hand(galn_hip_readset *rs)
     const int sl = (rs->seed_len > 0 && rs->seed_len < rs->max_len) ? rs->seed_len : 0;
     const uint32_t stride = (uint32_t)(1 + (rs->max_len + 1) + (sl + 1));
     const int cap = (int)std::min<uint64_t>((uint64_t)rs->n, HAND_BYTES / (stride * 8ull));
-    if (rs->d_hand && rs->hand_stride == stride && rs->hand_cap >= cap) return 0;
-    if (rs->d_hand) HIP_OK(hipFree(rs->d_hand));
-    rs->d_hand = nullptr;
-    rs->hand_cap = 0;
-    if (hipMalloc(&rs->d_hand, (size_t)std::max(cap, 1) * stride * 8) != hipSuccess) {
-        rs->d_hand = nullptr;  // no hand-off: the wave engine computes the widths itself
-        (void)hipGetLastError();
-        return 0;
+    // slots of this stride in the bytes already held (another launch's stride may differ)
+    const size_t need = (size_t)std::max(cap, 1) * stride * 8;
+    if (!rs->d_hand || rs->hand_bytes < need) {
+        if (grow_dev((void **)&rs->d_hand, rs->hand_bytes, need, rs->ctx->stream)) {
+            (void)hipGetLastError();
+            g_err[0] = 0;
+            rs->d_hand = nullptr;  // no hand-off: the wave engine computes the widths itself
+            rs->hand_bytes = 0;
+            rs->hand_cap = 0;
+            return 0;
+        }
     }
     rs->hand_stride = stride;
-    rs->hand_cap = cap;
+    rs->hand_cap = (int)std::min<uint64_t>((uint64_t)cap, rs->hand_bytes / (stride * 8ull));
     return 0;
 }
 
@@ -2657,11 +2670,14 @@ int run_tier(galn_hip_readset *rs, uint32_t flags, const Tier &t, const int32_t 
     hipStream_t fst = cx->stream;
     if (n_list <= 0) return 0;
     if (!persist) slots = n_list;
+    // scratch and pool for every lane of a full persistent grid, also when this launch has fewer
+    // reads, so that launches of varying size reuse them
+    const int alloc_lanes = std::max(1, slots);
     slots = std::max(1, std::min(slots, n_list));
     const int own_cap = (int)(((int64_t)(rs->max_entries + 16) / CH + MAXB + 2) / GRAN + 2);
     Layout L = make_layout(rs->max_len, rs->seed_len, t.rec_cap, own_cap);
     // scratch, and a pool for what the lanes keep plus room for the heavy reads in flight
-    int r = prepare_tier(rs, t, slots);
+    int r = prepare_tier(rs, t, alloc_lanes);
     if (r) return r;
     KArgs a = base_args(rs, flags);
     a.list = list;
@@ -3067,17 +3083,17 @@ int readset_reserve(galn_hip_readset *rs, int n, uint64_t qbytes, int nb)
     };
     bool ok = true;
     if (grow_q) {
-        const uint64_t c = std::max<uint64_t>(qbytes, rs->cap_q + rs->cap_q / 2);
+        const uint64_t c = std::max<uint64_t>(qbytes, 2 * rs->cap_q);
         ok = regrow((void **)&rs->d_query, c + 16);
         rs->cap_q = ok ? c : 0;
     }
     if (ok && grow_b) {
-        const int c = std::max(nb, rs->cap_b + rs->cap_b / 2);
+        const int c = std::max(nb, 2 * rs->cap_b);
         ok = regrow((void **)&rs->d_nstacks, (size_t)c * 4) && regrow((void **)&rs->d_bopts, (size_t)c * sizeof(galn_search_opt_t));
         rs->cap_b = ok ? c : 0;
     }
     if (ok && grow_n) {
-        const size_t c = (size_t)std::max<int64_t>(n, (int64_t)rs->cap_n + rs->cap_n / 2);
+        const size_t c = (size_t)std::min<int64_t>(std::max<int64_t>(n, 2 * (int64_t)rs->cap_n), INT32_MAX);
         ok = regrow((void **)&rs->d_qoff, c * 8) && regrow((void **)&rs->d_qlen, c * 4) &&
              regrow((void **)&rs->d_maxdiff, c * 4) && regrow((void **)&rs->d_batch, c * 4) &&
              regrow((void **)&rs->d_sorted, c * 4) && regrow((void **)&rs->d_out_n, c * 4) &&
@@ -3197,6 +3213,15 @@ __attribute__((visibility("hidden"))) int galn_hip_readset_make_private(galn_hip
     rs->own_ctx = c;
     rs->ctx = c;
     return 0;
+}
+
+// Capacity for launches of up to n reads, qbytes query bytes and nb batches, allocated now (a
+// server slot's read set, before its first launch fills it): later launches of up to that size
+// then allocate nothing per read. The read set's contents are not kept.
+__attribute__((visibility("hidden"))) int galn_hip_readset_reserve(galn_hip_readset *rs, int n, uint64_t qbytes, int nb)
+{
+    HIP_OK(hipSetDevice(rs->idx->device));
+    return readset_reserve(rs, std::max(n, 1), qbytes, std::max(nb, 1));
 }
 
 namespace {
@@ -3450,12 +3475,7 @@ int galn_hip_readset_results(galn_hip_readset *rs, galn_hip_out *out)
     size_t tmp_bytes = 0;
     const rocprim::transform_iterator<const int32_t *, RecCount, uint64_t> cnt(rs->d_out_n, RecCount());
     HIP_OK(rocprim::inclusive_scan(nullptr, tmp_bytes, cnt, rs->d_incl, (size_t)n, rocprim::plus<uint64_t>(), s));
-    if (tmp_bytes > rs->scan_tmp_bytes) {
-        if (rs->d_scan_tmp) HIP_OK(hipFree(rs->d_scan_tmp));
-        rs->d_scan_tmp = nullptr;
-        HIP_OK(hipMalloc(&rs->d_scan_tmp, tmp_bytes));
-        rs->scan_tmp_bytes = tmp_bytes;
-    }
+    if (int g = grow_dev(&rs->d_scan_tmp, rs->scan_tmp_bytes, tmp_bytes, s)) return g;
     HIP_OK(rocprim::inclusive_scan(rs->d_scan_tmp, tmp_bytes, cnt, rs->d_incl, (size_t)n, rocprim::plus<uint64_t>(), s));
     unsigned long long need = 0;
     HIP_OK(hipMemcpyAsync(&need, rs->d_incl + (n - 1), 8, hipMemcpyDeviceToHost, s));
@@ -3468,14 +3488,8 @@ int galn_hip_readset_results(galn_hip_readset *rs, galn_hip_out *out)
     if (need > out->rec_cap || (need > 0 && !out->rec))
         return fail(GALN_HIP_E_RECCAP, "record array holds %llu, %llu needed", (unsigned long long)out->rec_cap,
                     (unsigned long long)need);
-    if (rs->stage_cap < need) {
-        if (rs->d_stage) HIP_OK(hipFree(rs->d_stage));
-        rs->d_stage = nullptr;
-        rs->stage_cap = 0;
-        const uint64_t cap = std::max<uint64_t>(need, rs->rec_cap);
-        HIP_OK(hipMalloc(&rs->d_stage, cap * sizeof(galn_aln_rec_t)));
-        rs->stage_cap = cap;
-    }
+    if (int g = grow_dev((void **)&rs->d_stage, rs->stage_bytes, std::max<uint64_t>(need, 1) * sizeof(galn_aln_rec_t), s))
+        return g;
     hipLaunchKernelGGL(k_gather_records, dim3((n + 255) / 256), dim3(256), 0, s, rs->d_out_n, rs->d_out_off,
                        (const uint64_t *)rs->d_incl, (const galn_aln_rec_t *)rs->d_rec, rs->d_stage, rs->d_roff, n);
     HIP_OK(hipGetLastError());

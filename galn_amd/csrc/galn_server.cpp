@@ -83,6 +83,7 @@ void galn_hip_index_hold(galn_hip_index *idx, int delta);
 void galn_hip_index_serve(galn_hip_index *idx, int delta);
 void galn_hip_server_worker_mark();
 int galn_hip_readset_make_private(galn_hip_readset *rs);
+int galn_hip_readset_reserve(galn_hip_readset *rs, int n, uint64_t qbytes, int nb);
 
 namespace {
 
@@ -235,8 +236,11 @@ void worker_main(galn_hip_server *S, int slot)
         if (!r) {
             if (rs) {
                 r = galn_hip_readset_reload(rs, &R);
-            } else if (!(r = galn_hip_readset_upload(S->idx, &R, &rs))) {
-                r = galn_hip_readset_make_private(rs);
+            } else if (!(r = galn_hip_readset_upload(S->idx, &R, &rs)) && !(r = galn_hip_readset_make_private(rs))) {
+                // room for a full launch of reads up to 256 bp, then this launch's reads again
+                r = galn_hip_readset_reserve(rs, S->cfg.max_launch_reads, (uint64_t)S->cfg.max_launch_reads * 256,
+                                             std::max((int)bopts.size(), 1024));
+                if (!r) r = galn_hip_readset_reload(rs, &R);
             }
         }
         if (!r) r = galn_hip_readset_search(S->idx, rs, S->cfg.form, S->cfg.flags, nullptr);
