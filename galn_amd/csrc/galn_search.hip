@@ -3419,7 +3419,8 @@ int galn_hip_readset_autotune(galn_hip_index *idx, galn_hip_readset *rs, uint32_
         galn_hip_tune_t t{};
         t.sort_by_cost = 1;
         t.block_threads = 128;
-        const int cut[] = {128, 256, 512, 1024, 2048, 4096};
+        // (1: every read that pops goes to the wave engine, k_search computing only its widths)
+        const int cut[] = {1, 16, 64, 128, 256, 512, 1024, 4096};
         int r;
         if (heavy_ok(rs)) {
             for (int c : cut) {

@@ -243,7 +243,13 @@ void worker_main(galn_hip_server *S, int slot)
                 if (!r) r = galn_hip_readset_reload(rs, &R);
             }
         }
-        if (!r) r = galn_hip_readset_search(S->idx, rs, S->cfg.form, S->cfg.flags, nullptr);
+        if (!r) {
+            // the library's tuning, with the configured wave-engine cutoff when one is set
+            galn_hip_tune_t t{};
+            t.sort_by_cost = 1;
+            t.heavy_pops = S->cfg.heavy_pops;
+            r = galn_hip_readset_search(S->idx, rs, S->cfg.form, S->cfg.flags, S->cfg.heavy_pops ? &t : nullptr);
+        }
         n_aln.resize(m), off.resize(m), pops.resize(m), trace.resize(m), bytes.resize(m);
         galn_hip_out O{};
         O.n_aln = n_aln.data();

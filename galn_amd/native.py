@@ -79,7 +79,7 @@ class Tune(C.Structure):
 
 class ServerCfg(C.Structure):
     _fields_ = [("max_launch_reads", C.c_int32), ("quantum_reads", C.c_int32), ("max_queued_bytes", C.c_uint64),
-                ("flags", C.c_uint32), ("form", C.c_int32), ("slots", C.c_int32)]
+                ("flags", C.c_uint32), ("form", C.c_int32), ("slots", C.c_int32), ("heavy_pops", C.c_int32)]
 
 
 class GapOpt(C.Structure):
@@ -538,11 +538,11 @@ class Server:
     and searched in shared launches, up to `slots` launches on the device at once."""
 
     def __init__(self, index: Index, max_launch_reads: int = 0, quantum_reads: int = 0, max_queued_bytes: int = 0,
-                 flags: int = 0, form: int = FORM_BATCHED, slots: int = 0):
+                 flags: int = 0, form: int = FORM_BATCHED, slots: int = 0, heavy_pops: int = 0):
         cfg = ServerCfg()
         cfg.max_launch_reads, cfg.quantum_reads = int(max_launch_reads), int(quantum_reads)
         cfg.max_queued_bytes, cfg.flags, cfg.form = int(max_queued_bytes), int(flags), int(form)
-        cfg.slots = int(slots)
+        cfg.slots, cfg.heavy_pops = int(slots), int(heavy_pops)
         h = C.c_void_p()
         rc = hip_lib().galn_hip_server_open(index.h, C.byref(cfg), C.byref(h))
         if rc != 0:

@@ -28,7 +28,7 @@
 extern "C" {
 #endif
 
-#define GALN_HIP_ABI_VERSION 3
+#define GALN_HIP_ABI_VERSION 4
 
 /* ---- library return codes ---- */
 #define GALN_HIP_OK             0
@@ -139,6 +139,8 @@ typedef struct {
     int32_t form;              /* 1 = batched (default), 0 = plain */
     int32_t slots;             /* launches on the device at once, each with its own worker thread,
                                   stream and stack pool (default 3, at most 16) */
+    int32_t heavy_pops;        /* batched form: pops after which a read moves to the wave-per-read
+                                  engine, as galn_hip_tune_t.heavy_pops (0: the library default) */
 } galn_hip_server_cfg_t;
 
 typedef struct galn_hip_index galn_hip_index;

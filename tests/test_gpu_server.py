@@ -44,11 +44,12 @@ def requests(golden_dir):
     return w, out
 
 
-@pytest.mark.parametrize("slots", [1, 4])
-def test_requests_in_a_mix_equal_alone_and_reference(slots, golden_dir):
+@pytest.mark.parametrize("slots,heavy_pops", [(1, 0), (4, 0), (3, 1)])
+def test_requests_in_a_mix_equal_alone_and_reference(slots, heavy_pops, golden_dir):
     """With one launch slot, and with four (a request's reads then run in several launches at
-    once and may finish out of order), every request equals its reads searched alone."""
-    print(f"case server mix: 9 requests, 2 option sets, {slots} slots", flush=True)
+    once and may finish out of order), every request equals its reads searched alone; also with
+    every read that pops sent to the wave engine (cutoff 1)."""
+    print(f"case server mix: 9 requests, 2 option sets, {slots} slots, cutoff {heavy_pops}", flush=True)
     w, reqs = requests(golden_dir)
     meta, words = ix.read_bwt(w["bwt"])
     exp = w["expected"]
@@ -59,7 +60,8 @@ def test_requests_in_a_mix_equal_alone_and_reference(slots, golden_dir):
             rs.search(nv.FORM_BATCHED, nv.GALN_HIP_TRACE)
             alone.append(rs.results(want_pops=True, want_trace=True))
             rs.close()
-        with nv.Server(idx, max_launch_reads=512, quantum_reads=96, flags=nv.GALN_HIP_TRACE, slots=slots) as srv:
+        with nv.Server(idx, max_launch_reads=512, quantum_reads=96, flags=nv.GALN_HIP_TRACE, slots=slots,
+                       heavy_pops=heavy_pops) as srv:
             ids = [srv.submit(*arrays) for _, _, _, arrays in reqs]
             assert all(i is not None for i in ids)
             got = []
