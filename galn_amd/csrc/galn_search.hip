@@ -2622,14 +2622,19 @@ int prepare_heavy(galn_hip_readset *rs, int waves, int rec_cap)
 }
 
 // Hand-off slots for the widths of reads moving to the wave engine: one per read, up to
-// HAND_BYTES (reads beyond it recompute their widths there).
-constexpr uint64_t HAND_BYTES = 4ull << 30;
+// HAND_BYTES or a quarter of the free device memory (reads beyond recompute their widths there,
+// one lane walking the read while the wave waits: with every read sent to the wave engine, 10M
+// reads of up to 150 bp need 12 GB).
+constexpr uint64_t HAND_BYTES = 32ull << 30;
 
 int prepare_hand(galn_hip_readset *rs)
 {
     const int sl = (rs->seed_len > 0 && rs->seed_len < rs->max_len) ? rs->seed_len : 0;
     const uint32_t stride = (uint32_t)(1 + (rs->max_len + 1) + (sl + 1));
-    const int cap = (int)std::min<uint64_t>((uint64_t)rs->n, HAND_BYTES / (stride * 8ull));
+    size_t freeb = 0, totalb = 0;
+    HIP_OK(hipMemGetInfo(&freeb, &totalb));
+    const uint64_t limit = std::min<uint64_t>(HAND_BYTES, (uint64_t)(freeb + rs->hand_bytes) / 4);
+    const int cap = (int)std::min<uint64_t>((uint64_t)rs->n, limit / (stride * 8ull));
     // slots of this stride in the bytes already held (another launch's stride may differ)
     const size_t need = (size_t)std::max(cap, 1) * stride * 8;
     if (!rs->d_hand || rs->hand_bytes < need) {
