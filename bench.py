@@ -160,8 +160,8 @@ def measured_traffic(kernel: str, args, tuned: dict) -> int | None:
         f = ln.split()
         if len(f) == 3 and f[0] == "traffic_bytes_per_launch" and f[1] == kernel:
             value = int(f[2])
-        if len(f) == 5 and f[0] == "traffic_config":
-            config = {f[1]: int(f[2]), f[3]: int(f[4])}
+        if len(f) >= 3 and len(f) % 2 == 1 and f[0] == "traffic_config":
+            config = {f[i]: int(f[i + 1]) for i in range(1, len(f), 2)}
     if config is None or any(tuned.get(k) != v for k, v in config.items()):
         return None
     return value
@@ -352,6 +352,8 @@ def main() -> int:
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="also copy a buffer of this many MiB device to device three times (the known-size "
                          "kernel that scales the FETCH_SIZE / WRITE_SIZE counters)")
+    ap.add_argument("--tune", default="", metavar="JSON",
+                    help='use this tuning instead of autotuning, e.g. \'{"heavy_pops": 1, "block_threads": 128}\'')
     ap.add_argument("--autotune-reads", type=int, default=0,
                     help="reads the autotuner searches (the first ones; 0: all for adna/plain, 200k otherwise)")
     ap.add_argument("--end-to-end", action="store_true",
@@ -421,7 +423,11 @@ def main() -> int:
     BATCHED_FLAGS = nv.GALN_HIP_GRAPH
     at_n = args.autotune_reads or (w.n if args.workload in ("adna", "plain") else 200_000)
     at_n = min(at_n, w.n)
-    if at_n < w.n:
+    if args.tune:
+        # a given tuning (profiling passes that must all run the same one)
+        tuned = {k: int(v) for k, v in json.loads(args.tune).items()}
+        log("tuning given on the command line")
+    elif at_n < w.n:
         # tune on the first at_n reads, then search the whole read set with that tuning
         log(f"autotuning the batched form on the first {at_n} reads")
         hi_q = int((w.qoff[:at_n] + w.lens[:at_n].astype(np.uint64)).max())
@@ -436,7 +442,7 @@ def main() -> int:
     TUNE = nv.Tune()
     for k, v in tuned.items():
         setattr(TUNE, k, int(v))
-    log(f"autotuned: {tuned}")
+    log(f"tuning: {tuned}")
     # host arrays for the records, allocated once in pinned memory and reused by every step (as a
     # caller that searches batch after batch would): the copies then run at full PCIe speed
     def pinned(shape, dtype):
