@@ -355,7 +355,7 @@ def main() -> int:
     ap.add_argument("--tune", default="", metavar="JSON",
                     help='use this tuning instead of autotuning, e.g. \'{"heavy_pops": 1, "block_threads": 128}\'')
     ap.add_argument("--autotune-reads", type=int, default=0,
-                    help="reads the autotuner searches (the first ones; 0: all for adna/plain, 200k otherwise)")
+                    help="reads the autotuner searches (the first ones; 0: all the timed reads)")
     ap.add_argument("--end-to-end", action="store_true",
                     help="also time `galn aln` on the reads as a gzip FASTQ, streamed and whole-input")
     ap.add_argument("--workdir", default="")
@@ -421,7 +421,10 @@ def main() -> int:
     # the batched form: persistent kernel + device work queue, launches replayed as a captured
     # hipGraph, tuning chosen by the library's autotuner on this read set
     BATCHED_FLAGS = nv.GALN_HIP_GRAPH
-    at_n = args.autotune_reads or (w.n if args.workload in ("adna", "plain") else 200_000)
+    # tuned on the read set that is timed: on a smaller sample of the repeat-rich workloads the
+    # longest single searches decide every trial (on the first 200k human-scale reads cutoff 1 won
+    # by 8%; on the 2M timed reads it was 1.8x slower than cutoff 128)
+    at_n = args.autotune_reads or w.n
     at_n = min(at_n, w.n)
     if args.tune:
         # a given tuning (profiling passes that must all run the same one)

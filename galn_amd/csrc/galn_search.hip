@@ -2425,7 +2425,8 @@ uint64_t pool_test_cap()
     return (uint64_t)std::max(1L, strtol(v, nullptr, 10));
 }
 
-// The stack pool: `want` granules of 64 chunks (16 KB each), within half the free memory.
+// The stack pool: `want` granules of 64 chunks (16 KB each), within half the free memory and the
+// 28-bit chunk ids of the bucket tops (2^22 granules, 64 GiB).
 int ensure_pool(SearchCtx *idx, uint64_t want)
 {
     const uint64_t gbytes = (uint64_t)GRAN * CH * sizeof(Ent);
@@ -2433,7 +2434,6 @@ int ensure_pool(SearchCtx *idx, uint64_t want)
     HIP_OK(hipMemGetInfo(&freeb, &totalb));
     const uint64_t have = idx->n_gran;
     uint64_t cap = (uint64_t)((freeb + have * (gbytes + GRAN * 4 + 4)) / 2) / (gbytes + GRAN * 4 + 4);
-    cap = std::min<uint64_t>(cap, ((uint64_t)48 << 30) / gbytes);
     cap = std::min<uint64_t>(cap, MAX_POOL_CHUNKS / GRAN);
     cap = std::min<uint64_t>(cap, pool_test_cap());
     want = std::min(std::max<uint64_t>(want, 1024), cap);
@@ -2614,7 +2614,11 @@ int prepare_tier(galn_hip_readset *rs, const Tier &t, int slots)
 int prepare_heavy(galn_hip_readset *rs, int waves, int rec_cap)
 {
     SearchCtx *cx = rs->ctx;
-    int r = ensure_pool(cx, (uint64_t)waves * 256);
+    // the largest pool there is room for: one wave's read can hold millions of entries (repeat-rich
+    // genomes), and once the never-used granules are gone every allocation searches the other
+    // shards' free lists (on the 3.1 Gbp human-like workload a 26 GB pool spent 3x the search
+    // time allocating)
+    int r = ensure_pool(cx, ~0ull);
     if (r) return r;
     const int own_cap = (int)std::min<uint64_t>(cx->n_gran, 32768);
     const HLayout L = make_hlayout(rec_cap, own_cap);
