@@ -2248,6 +2248,7 @@ struct SearchCtx {
     uint32_t *d_pool_link = nullptr, *d_gnext = nullptr, *d_gbump = nullptr;
     unsigned long long *d_ghead = nullptr;
     uint32_t n_gran = 0;
+    bool grow = false;  // the last search used every never-used granule: double the pool next time
     // frees the pool (not the stream)
     void free_pool()
     {
@@ -2429,6 +2430,10 @@ uint64_t pool_test_cap()
 // 28-bit chunk ids of the bucket tops (2^22 granules, 64 GiB).
 int ensure_pool(SearchCtx *idx, uint64_t want)
 {
+    // contexts of one process size their pools one at a time (server slots start together: each
+    // would otherwise count the same free memory)
+    static std::mutex mu;
+    std::lock_guard<std::mutex> lk(mu);
     const uint64_t gbytes = (uint64_t)GRAN * CH * sizeof(Ent);
     size_t freeb = 0, totalb = 0;
     HIP_OK(hipMemGetInfo(&freeb, &totalb));
@@ -2436,6 +2441,11 @@ int ensure_pool(SearchCtx *idx, uint64_t want)
     uint64_t cap = (uint64_t)((freeb + have * (gbytes + GRAN * 4 + 4)) / 2) / (gbytes + GRAN * 4 + 4);
     cap = std::min<uint64_t>(cap, MAX_POOL_CHUNKS / GRAN);
     cap = std::min<uint64_t>(cap, pool_test_cap());
+    // a search that used up the never-used granules made every later allocation search the other
+    // shards' free lists (one wave's read can hold millions of entries on a repeat-rich genome: on
+    // the 3.1 Gbp human-like workload a 26 GB pool spent 3x the search time allocating)
+    if (idx->grow) want = std::max<uint64_t>(want, 2 * have);
+    idx->grow = false;
     want = std::min(std::max<uint64_t>(want, 1024), cap);
     if (have >= want) return 0;
     if (!idx->d_ghead) {
@@ -2461,13 +2471,20 @@ int ensure_pool(SearchCtx *idx, uint64_t want)
         if (p) HIP_OK(hipFree(p));
     Ent *chk = nullptr;
     uint32_t *link = nullptr, *gnext = nullptr;
-    if (hipMalloc(&chk, want * gbytes) != hipSuccess || hipMalloc(&link, want * GRAN * 4) != hipSuccess ||
-        hipMalloc(&gnext, want * 4) != hipSuccess) {
+    // half as many granules while the allocation fails (other allocations of the process, or of
+    // other processes, since the free memory was read): a smaller pool only costs re-runs
+    for (;;) {
+        if (hipMalloc(&chk, want * gbytes) == hipSuccess && hipMalloc(&link, want * GRAN * 4) == hipSuccess &&
+            hipMalloc(&gnext, want * 4) == hipSuccess)
+            break;
         (void)hipGetLastError();
         for (void *p : {(void *)chk, (void *)link, (void *)gnext})
             if (p) (void)hipFree(p);
-        return fail(GALN_HIP_E_NOMEM, "stack pool of %llu granules does not fit in device memory",
-                    (unsigned long long)want);
+        chk = nullptr, link = nullptr, gnext = nullptr;
+        if (want <= 1024)
+            return fail(GALN_HIP_E_NOMEM, "stack pool of %llu granules does not fit in device memory",
+                        (unsigned long long)want);
+        want = std::max<uint64_t>(want / 2, 1024);
     }
     idx->d_pool_chk = chk;
     idx->d_pool_link = link;
@@ -2601,24 +2618,27 @@ KArgs base_args(galn_hip_readset *rs, uint32_t flags)
 // One launch over list positions [0, n_list) (read id = list[pos], or list_base + pos);
 // failures land in fail buffer `fo`.
 // The allocations a first-tier launch over `slots` lanes needs (run before a graph capture).
-int prepare_tier(galn_hip_readset *rs, const Tier &t, int slots)
+int prepare_tier(galn_hip_readset *rs, const Tier &t, int slots, int pool_lanes)
 {
     const int own_cap = (int)(((int64_t)(rs->max_entries + 16) / CH + MAXB + 2) / GRAN + 2);
     const Layout L = make_layout(rs->max_len, rs->seed_len, t.rec_cap, own_cap);
     int r = ensure_scratch(rs, (size_t)slots * L.slot_bytes);
     if (r) return r;
-    return ensure_pool(rs->ctx, (uint64_t)slots * (uint64_t)(t.keep + 12));
+    // the pool for the lanes that get reads, rounded up to a power of two so that launches of
+    // varying size regrow it a few times at most
+    uint64_t want = (uint64_t)std::max(pool_lanes, 1) * (uint64_t)(t.keep + 12), p2 = 1024;
+    while (p2 < want) p2 <<= 1;
+    return ensure_pool(rs->ctx, p2);
 }
 
 // The allocations of a wave-engine launch of `waves` waves.
 int prepare_heavy(galn_hip_readset *rs, int waves, int rec_cap)
 {
     SearchCtx *cx = rs->ctx;
-    // the largest pool there is room for: one wave's read can hold millions of entries (repeat-rich
-    // genomes), and once the never-used granules are gone every allocation searches the other
-    // shards' free lists (on the 3.1 Gbp human-like workload a 26 GB pool spent 3x the search
-    // time allocating)
-    int r = ensure_pool(cx, ~0ull);
+    // granules for the waves that can get reads, rounded up to a power of two
+    uint64_t want = (uint64_t)std::max(1, std::min(waves, rs->n)) * 256, p2 = 1024;
+    while (p2 < want) p2 <<= 1;
+    int r = ensure_pool(cx, p2);
     if (r) return r;
     const int own_cap = (int)std::min<uint64_t>(cx->n_gran, 32768);
     const HLayout L = make_hlayout(rec_cap, own_cap);
@@ -2686,7 +2706,7 @@ int run_tier(galn_hip_readset *rs, uint32_t flags, const Tier &t, const int32_t 
     const int own_cap = (int)(((int64_t)(rs->max_entries + 16) / CH + MAXB + 2) / GRAN + 2);
     Layout L = make_layout(rs->max_len, rs->seed_len, t.rec_cap, own_cap);
     // scratch, and a pool for what the lanes keep plus room for the heavy reads in flight
-    int r = prepare_tier(rs, t, alloc_lanes);
+    int r = prepare_tier(rs, t, alloc_lanes, slots);
     if (r) return r;
     KArgs a = base_args(rs, flags);
     a.list = list;
@@ -2907,8 +2927,14 @@ int run_retries(galn_hip_readset *rs, uint32_t flags, int fi, int32_t n, Tier t,
         if (kind[0] && (r = grow_records(rs))) return r;
         if (kind[1]) t.rec_cap *= 16;
         if (kind[2]) {
-            slots = std::max(block, slots / 4);
-            hwaves = std::max(1, hwaves / 4);
+            // an exhausted pool: twice the granules when there is room, else fewer lanes sharing it
+            const uint32_t before = rs->ctx->n_gran;
+            rs->ctx->grow = true;
+            if ((r = ensure_pool(rs->ctx, before))) return r;
+            if (rs->ctx->n_gran <= before) {
+                slots = std::max(block, slots / 4);
+                hwaves = std::max(1, hwaves / 4);
+            }
         }
         slots = std::min(slots, std::max(block, (int)n));
         const int fo = fi ^ 1;
@@ -2985,7 +3011,9 @@ int readset_search(galn_hip_readset *rs, int form, uint32_t flags, const galn_hi
             key.flags = flags, key.block = block, key.slots = slots, key.hwaves = hwaves, key.n = rs->n;
             key.budget = budget, key.rec_cap = (uint32_t)t0.rec_cap, key.keep = (uint32_t)t0.keep, key.list = list;
             key.push_cap = heavy_push_cap();
-            if ((r = prepare_tier(rs, t0, slots)) || (budget && (r = prepare_heavy(rs, hwaves, 256)))) return r;
+            if ((r = prepare_tier(rs, t0, slots, std::min(slots, rs->n))) ||
+                (budget && (r = prepare_heavy(rs, hwaves, 256))))
+                return r;
             while (rs->lev.size() < 4) {
                 hipEvent_t e;
                 HIP_OK(hipEventCreate(&e));
@@ -3050,6 +3078,12 @@ int readset_search(galn_hip_readset *rs, int form, uint32_t flags, const galn_hi
     if ((r = read_ctr(rs, 0, &nf))) return r;
     if (nf > 0 && (r = run_retries(rs, flags, 0, nf, t0, slots, budget, std::max(1, hwaves)))) return r;
     HIP_OK(hipStreamSynchronize(fst));
+    if (rs->ctx->d_gbump) {
+        uint32_t used = 0;
+        HIP_OK(hipMemcpyAsync(&used, rs->ctx->d_gbump, 4, hipMemcpyDeviceToHost, fst));
+        HIP_OK(hipStreamSynchronize(fst));
+        if (used >= rs->ctx->n_gran) rs->ctx->grow = true;
+    }
     std::vector<unsigned long long> ab(rs->launches.size());
     if (!ab.empty())
         HIP_OK(hipMemcpy(ab.data(), rs->d_algo, ab.size() * sizeof(unsigned long long), hipMemcpyDeviceToHost));
