@@ -1421,10 +1421,13 @@ __global__ void __launch_bounds__(256) k_widths(WArgs a)
     }
 }
 
-template <bool TRACE>
-// 4 waves per SIMD (16 per CU): at 128 VGPRs the compiler spills a few values to scratch, and
-// the extra resident waves hide more of the index latency than the spills cost (measured 3.4%)
-__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) k_heavy(HArgs a)
+// WPE waves per SIMD: 4 (16 per CU) at 128 VGPRs, the compiler spilling a few values to scratch; the
+// extra resident waves hide more of the index latency than the spills cost (measured 3.4% against 3
+// per SIMD on the 100 Mbp workload, where 5 was 10% slower: that index sits in the MALL and the
+// engine is bound by instruction issue). 5 (96 VGPRs, more spills) for indexes in HBM, where a wave
+// waits on its loads: 20% faster on the 3.1 Gbp human-like workload, 25% slower on the 20 Gbp one.
+template <bool TRACE, int WPE>
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WPE))) k_heavy(HArgs a)
 {
     extern __shared__ __align__(16) uint8_t hlds[];
     HShared &F = *reinterpret_cast<HShared *>(hlds);
@@ -2288,6 +2291,7 @@ struct GraphKey {
     int32_t hand_cap;
     uint32_t hand_stride;
     int32_t push_cap;
+    int32_t heavy_wpe;
 };
 
 struct galn_hip_readset {
@@ -2344,6 +2348,7 @@ struct galn_hip_readset {
     bool has_tuned = false;
     galn_hip_tune_t tuned{};
     uint32_t last_flags = 0;
+    int heavy_wpe = 4;  // waves per SIMD of the wave engine in the current search (4 or 5)
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
     struct Launch {
         int tier;
@@ -2863,10 +2868,18 @@ int run_heavy(galn_hip_readset *rs, uint32_t flags, const int32_t *list, int n, 
         hipLaunchKernelGGL(k_widths, dim3(wgrid), dim3(256), 0, fst, w);
         HIP_OK(hipGetLastError());
     }
-    if (flags & GALN_HIP_TRACE)
-        hipLaunchKernelGGL(k_heavy<true>, dim3(waves), dim3(HW), lds, fst, a);
-    else
-        hipLaunchKernelGGL(k_heavy<false>, dim3(waves), dim3(HW), lds, fst, a);
+    const bool tr = (flags & GALN_HIP_TRACE) != 0;
+    if (rs->heavy_wpe == 5) {
+        if (tr)
+            hipLaunchKernelGGL((k_heavy<true, 5>), dim3(waves), dim3(HW), lds, fst, a);
+        else
+            hipLaunchKernelGGL((k_heavy<false, 5>), dim3(waves), dim3(HW), lds, fst, a);
+    } else {
+        if (tr)
+            hipLaunchKernelGGL((k_heavy<true, 4>), dim3(waves), dim3(HW), lds, fst, a);
+        else
+            hipLaunchKernelGGL((k_heavy<false, 4>), dim3(waves), dim3(HW), lds, fst, a);
+    }
     HIP_OK(hipGetLastError());
     if (!rs->capturing) HIP_OK(hipEventRecord(rs->lev[2 * li + 1], fst));
     return 0;
@@ -2878,8 +2891,14 @@ int heavy_waves(galn_hip_readset *rs, uint32_t flags)
     const int swlen = (rs->seed_len > 0 && rs->seed_len < rs->max_len) ? rs->seed_len + 1 : 1;
     const size_t lds = heavy_lds_bytes(buckets_of(rs), wlen, swlen);
     int nb = 0;
-    hipError_t e = (flags & GALN_HIP_TRACE) ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_heavy<true>, HW, lds)
-                                            : hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_heavy<false>, HW, lds);
+    const bool tr = (flags & GALN_HIP_TRACE) != 0;
+    hipError_t e;
+    if (rs->heavy_wpe == 5)
+        e = tr ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_heavy<true, 5>, HW, lds)
+               : hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_heavy<false, 5>, HW, lds);
+    else
+        e = tr ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_heavy<true, 4>, HW, lds)
+               : hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_heavy<false, 4>, HW, lds);
     if (e != hipSuccess || nb < 1) nb = 1;
     return nb * rs->idx->n_cu;
 }
@@ -2999,6 +3018,7 @@ int readset_search(galn_hip_readset *rs, int form, uint32_t flags, const galn_hi
         slots = resident_slots(rs, block, trace, tune ? tune->waves_per_cu : 0);
         const bool sorted = !tune || tune->sort_by_cost != 0;
         const int hp = tune ? tune->heavy_pops : 0;
+        rs->heavy_wpe = tune && tune->heavy_waves_per_simd == 5 ? 5 : 4;
         budget = (hp < 0 || !heavy_ok(rs)) ? 0 : (uint64_t)(hp > 0 ? hp : DEFAULT_HEAVY_POPS);
         hwaves = heavy_waves(rs, flags);
         if (budget && (r = prepare_hand(rs))) return r;
@@ -3011,6 +3031,7 @@ int readset_search(galn_hip_readset *rs, int form, uint32_t flags, const galn_hi
             key.flags = flags, key.block = block, key.slots = slots, key.hwaves = hwaves, key.n = rs->n;
             key.budget = budget, key.rec_cap = (uint32_t)t0.rec_cap, key.keep = (uint32_t)t0.keep, key.list = list;
             key.push_cap = heavy_push_cap();
+            key.heavy_wpe = rs->heavy_wpe;
             if ((r = prepare_tier(rs, t0, slots, std::min(slots, rs->n))) ||
                 (budget && (r = prepare_heavy(rs, hwaves, 256))))
                 return r;
@@ -3448,8 +3469,9 @@ int galn_hip_readset_autotune(galn_hip_index *idx, galn_hip_readset *rs, uint32_
             const double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
             if (verbose)
                 fprintf(stderr, "[galn_hip] autotune: cutoff %d, %d-thread groups, %d waves/CU, %d record slots, "
-                                "%d chunks kept: %.1f ms\n",
-                        t.heavy_pops, t.block_threads, t.waves_per_cu, t.rec_per_lane, t.chunks_per_lane, ms);
+                                "%d chunks kept, wave engine at %d waves/SIMD: %.1f ms\n",
+                        t.heavy_pops, t.block_threads, t.waves_per_cu, t.rec_per_lane, t.chunks_per_lane,
+                        t.heavy_waves_per_simd == 5 ? 5 : 4, ms);
             if (best_ms == 0 || ms < best_ms) {
                 best_ms = ms;
                 best = t;
@@ -3492,6 +3514,21 @@ int galn_hip_readset_autotune(galn_hip_index *idx, galn_hip_readset *rs, uint32_
             t = best;
             t.chunks_per_lane = v;
             if ((r = trial(t))) return r;
+        }
+        if (heavy_ok(rs) && best.heavy_pops >= 0) {
+            t = best;
+            t.heavy_waves_per_simd = 5;
+            if ((r = trial(t))) return r;
+            // with more waves in flight the best cutoff can move: the small ones again
+            if (best.heavy_waves_per_simd == 5) {
+                const galn_hip_tune_t at5 = best;
+                for (int c : {1, 16, 64, 128}) {
+                    if (c == at5.heavy_pops) continue;
+                    t = at5;
+                    t.heavy_pops = c;
+                    if ((r = trial(t))) return r;
+                }
+            }
         }
     }
     rs->tuned = best;

@@ -74,7 +74,7 @@ class Limits(C.Structure):
 
 class Tune(C.Structure):
     _fields_ = [(n, C.c_int32) for n in ("block_threads", "waves_per_cu", "chunks_per_lane", "rec_per_lane",
-                                         "sort_by_cost", "heavy_pops")]
+                                         "sort_by_cost", "heavy_pops", "heavy_waves_per_simd")]
 
 
 class ServerCfg(C.Structure):

@@ -28,7 +28,7 @@
 extern "C" {
 #endif
 
-#define GALN_HIP_ABI_VERSION 4
+#define GALN_HIP_ABI_VERSION 5
 
 /* ---- library return codes ---- */
 #define GALN_HIP_OK             0
@@ -124,6 +124,9 @@ typedef struct {
     int32_t sort_by_cost;      /* 1: claim reads longest-first */
     int32_t heavy_pops;        /* pops after which a read moves to the wave-per-read engine
                                   (0: default 512, -1: never) */
+    int32_t heavy_waves_per_simd; /* wave-per-read engine occupancy: 4 (0: default) or 5 (fewer
+                                  registers, more of them spilled; more loads in flight, which pays
+                                  for indexes in HBM) */
 } galn_hip_tune_t;
 
 /* Serving loop configuration (0 = default). */

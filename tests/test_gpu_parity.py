@@ -2,7 +2,7 @@
 
 Every golden run (the reference's own C search, committed in tests/golden/) is searched
 through the C ABI in each form -- plain, batched, batched with every read sent to the
-wave-per-read engine, batched without the exact-child register path, batched with one
+wave-per-read engine (also at 5 waves per SIMD), batched without the exact-child register path, batched with one
 record slot per lane (forces the re-run path) -- and must give the golden .sai bytes, pop
 counts and trace hashes. The `galn` executable must write the golden .sai. Larger inputs
 are checked by size-independent properties: the forms agree bit for bit, and a sample
@@ -36,6 +36,7 @@ FORMS = {
     "batched": (nv.FORM_BATCHED, 0, None),
     "heavy_all": (nv.FORM_BATCHED, 0, dict(heavy_pops=1)),
     "heavy_64": (nv.FORM_BATCHED, 0, dict(heavy_pops=64)),
+    "heavy_all_5_waves": (nv.FORM_BATCHED, 0, dict(heavy_pops=1, heavy_waves_per_simd=5)),
     "no_fuse": (nv.FORM_BATCHED, nv.GALN_HIP_NOFUSE, dict(heavy_pops=-1)),
     "rec1": (nv.FORM_BATCHED, 0, dict(rec_per_lane=1, heavy_pops=-1)),
     "unsorted_small_grid": (nv.FORM_BATCHED, 0, dict(sort_by_cost=0, waves_per_cu=1)),

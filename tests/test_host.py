@@ -46,7 +46,7 @@ def test_ctypes_mirror_binds_every_declared_function(lib, mirrored):
 
 
 def test_abi_version_and_limits():
-    assert nv.hip_lib().galn_hip_abi_version() == 4
+    assert nv.hip_lib().galn_hip_abi_version() == 5
     lim = nv.limits()
     assert lim["max_read_len"] == 1023 and lim["max_n_stacks"] == 256 and lim["max_seq_len"] == (1 << 38) - 1
 
