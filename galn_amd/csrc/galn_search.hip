@@ -2431,6 +2431,15 @@ uint64_t pool_test_cap()
     return (uint64_t)std::max(1L, strtol(v, nullptr, 10));
 }
 
+// Granules of a context's first pool: as asked, or the value of GALN_HIP_TEST_POOL_START, which the
+// tests set low so that the pool's growth on demand runs.
+uint64_t pool_test_start()
+{
+    const char *v = getenv("GALN_HIP_TEST_POOL_START");
+    if (!v || !*v) return ~0ull;
+    return (uint64_t)std::max(1024L, strtol(v, nullptr, 10));
+}
+
 // The stack pool: `want` granules of 64 chunks (16 KB each), within half the free memory and the
 // 28-bit chunk ids of the bucket tops (2^22 granules, 64 GiB).
 int ensure_pool(SearchCtx *idx, uint64_t want)
@@ -2451,6 +2460,7 @@ int ensure_pool(SearchCtx *idx, uint64_t want)
     // the 3.1 Gbp human-like workload a 26 GB pool spent 3x the search time allocating)
     if (idx->grow) want = std::max<uint64_t>(want, 2 * have);
     idx->grow = false;
+    if (have == 0) want = std::min(want, pool_test_start());
     want = std::min(std::max<uint64_t>(want, 1024), cap);
     if (have >= want) return 0;
     if (!idx->d_ghead) {

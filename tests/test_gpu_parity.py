@@ -307,6 +307,35 @@ def test_exhausted_pool_reruns_reads_with_fewer_lanes(form, golden_dir, monkeypa
     assert any(t not in (0, 10) for t in tiers), f"no re-run launch: {tiers}"
 
 
+def test_pool_grows_on_demand_and_results_stay_golden(golden_dir, monkeypatch):
+    """A context whose first pool holds only 1024 granules (GALN_HIP_TEST_POOL_START) runs out in
+    the first search: the pool doubles before the reads that found it empty are re-run, and again
+    after each search that used up its never-used granules, so later searches need fewer re-runs.
+    Every search gives the reference's records, pops and traces."""
+    print("case pool growth adna_mixed/adna_1", flush=True)
+    monkeypatch.setenv("GALN_HIP_TEST_POOL_START", "1024")
+    w = golden_inputs("adna_mixed", golden_dir)
+    opts, mate = run_opts(w["meta"], "adna_1")
+    r = load_reads(w["reads"][mate])
+    o = nv.gap_opt_from_args(opts)
+    bopts, nst, md = nv.prepare_batches(o, r["qlen"], r["batch"])
+    meta, words = ix.read_bwt(w["bwt"])
+    exp = w["expected"]
+    launches = []
+    with nv.Index(words, meta, 0) as idx:  # a fresh index: a fresh pool
+        rs = nv.ReadSet(idx, r["query"], r["qoff"], r["qlen"], md, r["batch"], bopts, nst)
+        for _ in range(3):
+            rs.search(nv.FORM_BATCHED, nv.GALN_HIP_TRACE, tune(heavy_pops=1))
+            res = rs.results(want_pops=True, want_trace=True)
+            launches.append(len(rs.launches()))
+            np.testing.assert_array_equal(res.n_aln, exp["adna_1_n_aln"])
+            np.testing.assert_array_equal(res.pops, exp["adna_1_pops"])
+            np.testing.assert_array_equal(res.trace, exp["adna_1_trace"])
+        rs.close()
+    assert launches[0] > 2, f"the first search did not run out of granules: {launches}"
+    assert launches[-1] <= launches[0], launches
+
+
 def test_index_close_is_refused_while_a_read_set_is_open(golden_dir):
     """galn_hip_index_close returns GALN_HIP_E_BUSY and keeps the index while one of its read
     sets is open (freeing the read set afterwards used its index's stream), and succeeds once
