@@ -4,8 +4,8 @@
 // The reference has no serving mode (it is a one-shot CLI, src/galn.cu:146-178); this is the
 // request front end for the same search. A request is a galn_hip_reads with its own options.
 //
-// Launch slots: each slot is a worker thread with its own device read set, stream and stack pool,
-// so up to `slots` launches are on the device together. A launch is bounded by its longest
+// Launch slots: each slot is a worker thread with its own device read set, stream and stack pool
+// (slot 0 those of the index), so up to `slots` launches are on the device together. A launch is bounded by its longest
 // search (a read that backtracks millions of entries keeps one wave busy while the rest of the
 // GPU has finished), and the next slot's launch fills the device meanwhile: requests that arrive
 // during a launch's tail start at once instead of waiting for it.
@@ -236,7 +236,13 @@ void worker_main(galn_hip_server *S, int slot)
         if (!r) {
             if (rs) {
                 r = galn_hip_readset_reload(rs, &R);
-            } else if (!(r = galn_hip_readset_upload(S->idx, &R, &rs)) && !(r = galn_hip_readset_make_private(rs))) {
+            } else if (!(r = galn_hip_readset_upload(S->idx, &R, &rs)) &&
+                       !(r = slot == 0 ? 0 : galn_hip_readset_make_private(rs))) {
+                // slot 0 keeps the index's own stream and pool (no one else may search the index
+                // while the server holds it): one stream fewer, and HIP maps streams onto 4
+                // hardware queues by default, so with a stream per slot two slots shared one and
+                // their launches ran one after the other (measured: 3.2-3.6e6 reads/s peak on the
+                // 100 Mbp index, 4.3e6 with slot 0 on the index's stream)
                 // room for a full launch of reads up to 256 bp, then this launch's reads again
                 r = galn_hip_readset_reserve(rs, S->cfg.max_launch_reads, (uint64_t)S->cfg.max_launch_reads * 256,
                                              std::max((int)bopts.size(), 1024));
