@@ -167,7 +167,8 @@ def measured_traffic(kernel: str, args, tuned: dict) -> int | None:
     return value
 
 
-def serving_sweep(w, index, nv, loads, seconds: float, max_req: int, seed: int = 7, slots: int = 0) -> dict:
+def serving_sweep(w, index, nv, loads, seconds: float, max_req: int, seed: int = 7, slots: int = 0,
+                  launch_reads: int = 0) -> dict:
     """Serving loop under Poisson arrivals. Requests are slices of the workload's reads of
     log-uniform size (1000 .. max_req reads), each searched with BWA's default options or the
     aDNA options (50/50), batch options computed per request as `galn aln` would. The
@@ -234,7 +235,7 @@ def serving_sweep(w, index, nv, loads, seconds: float, max_req: int, seed: int =
 
     out = {"request_reads": f"log-uniform 1000..{max_req}", "options": "50/50 default / aln -l 1024 -n 0.01 -o 2",
            "loads": []}
-    with nv.Server(index, slots=slots) as srv:
+    with nv.Server(index, slots=slots, max_launch_reads=launch_reads) as srv:
         warm = [make_request() for _ in range(2)]
         run(srv, warm, None)
         # the sustained rate over enough requests to keep every launch slot busy
@@ -349,6 +350,7 @@ def main() -> int:
     ap.add_argument("--serve-seconds", type=float, default=8.0, help="arrival time per offered load (0: no serving)")
     ap.add_argument("--serve-max-request", type=int, default=1_000_000)
     ap.add_argument("--serve-slots", type=int, default=0, help="serving launches on the device at once (0: library default)")
+    ap.add_argument("--serve-launch-reads", type=int, default=0, help="most reads per serving launch (0: library default)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="also copy a buffer of this many MiB device to device three times (the known-size "
                          "kernel that scales the FETCH_SIZE / WRITE_SIZE counters)")
@@ -573,7 +575,8 @@ def main() -> int:
     if args.serve_seconds > 0 and args.workload in ("adna", "human"):
         pool = serve_pool if serve_pool is not None else w
         out["serving"] = serving_sweep(pool, index, nv, (0.25, 0.5, 0.75, 0.9), args.serve_seconds,
-                                       min(args.serve_max_request, pool.n), slots=args.serve_slots)
+                                       min(args.serve_max_request, pool.n), slots=args.serve_slots,
+                                       launch_reads=args.serve_launch_reads)
         out["serving"]["peak_over_batched"] = out["serving"]["peak_reads_per_s"] / value
     rs.close()
     index.close()

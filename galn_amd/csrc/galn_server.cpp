@@ -320,7 +320,11 @@ int galn_hip_server_open(galn_hip_index *idx, const galn_hip_server_cfg_t *cfg, 
     galn_hip_server *S = new galn_hip_server;
     S->idx = idx;
     if (cfg) S->cfg = *cfg;
-    if (S->cfg.max_launch_reads <= 0) S->cfg.max_launch_reads = 262144;
+    // 1M-read launches: a launch waits for its longest searches, and with the slots overlapping
+    // larger launches waste less of the device on those tails (100 Mbp index: peak 4.70e6 reads/s
+    // against 3.51e6 at 262,144, p99 at half load 328 ms against 326); small requests still ride
+    // along with large ones (each waiting request gets quantum_reads of every launch first)
+    if (S->cfg.max_launch_reads <= 0) S->cfg.max_launch_reads = 1 << 20;
     if (S->cfg.quantum_reads <= 0) S->cfg.quantum_reads = 16384;
     S->cfg.quantum_reads = std::min(S->cfg.quantum_reads, S->cfg.max_launch_reads);
     if (S->cfg.max_queued_bytes == 0) S->cfg.max_queued_bytes = 4ull << 30;

@@ -131,7 +131,7 @@ typedef struct {
 
 /* Serving loop configuration (0 = default). */
 typedef struct {
-    int32_t max_launch_reads;  /* reads searched together in one launch (default 262144) */
+    int32_t max_launch_reads;  /* reads searched together in one launch (default 1048576) */
     int32_t quantum_reads;     /* reads a request gets per launch before the next request's
                                   reads are taken (default 16384) */
     uint64_t max_queued_bytes; /* admission: device bytes of the requests not yet collected
