@@ -141,7 +141,10 @@ typedef struct {
     uint32_t flags;            /* search flags of every launch */
     int32_t form;              /* 1 = batched (default), 0 = plain */
     int32_t slots;             /* launches on the device at once, each with its own worker thread,
-                                  stream and stack pool (default 3, at most 16) */
+                                  stream and stack pool; slot 0 uses the index's (default 3, at
+                                  most 16; HIP spreads streams over GPU_MAX_HW_QUEUES hardware
+                                  queues, 4 by default, and slots whose streams share one run
+                                  one after the other) */
     int32_t heavy_pops;        /* batched form: pops after which a read moves to the wave-per-read
                                   engine, as galn_hip_tune_t.heavy_pops (0: the library default) */
 } galn_hip_server_cfg_t;
