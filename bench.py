@@ -258,6 +258,7 @@ def serving_sweep(w, index, nv, loads, seconds: float, max_req: int, seed: int =
             log(f"serving: load {load:.2f}: {n_req} requests, p50 {rec['p50_ms']:.1f} ms, p99 {rec['p99_ms']:.1f} ms, "
                 f"{rec['reads_per_s']:.3e} reads/s")
         out["server_stats"] = srv.stats()
+        log(f"serving: server counters {out['server_stats']}")
     # the batched form on exactly the peak phase's requests (one read set, each request's reads
     # with its own batch options): the throughput the serving peak is to be compared with
     parts = list(zip(*[r for r in peak_reqs]))

@@ -114,6 +114,11 @@ struct galn_hip_server {
     std::vector<galn_hip_readset *> slot_rs;
     double stats[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     int busy_slots = 0;
+    // launch size: between launch_lo and launch_hi reads (both max_launch_reads when it is set),
+    // the reads the server finished in LAUNCH_SECONDS of recent busy time
+    int launch_lo = 0, launch_hi = 0, launch_reads = 0;
+    double rate_reads = 0, rate_sec = 0;  // reads finished and busy seconds, older launches decayed
+    Clock::time_point busy_mark;          // start of the busy time not yet counted
     // GALN_HIP_TEST_SERVER_FAIL=n: the n-th launch (from 1) fails on the host before it reaches
     // the device, so that tests can drive the error path of a request spread over several launches
     long fail_at = 0;
@@ -122,11 +127,20 @@ struct galn_hip_server {
 
 namespace {
 
+// A launch of the adaptive size holds the reads the server finished in this many seconds of
+// recent busy time. Larger launches waste less of the device on a launch's tail where the longest
+// searches take a fraction of a second (100 Mbp index, ~4e6 reads/s: peak 4.70e6 reads/s with
+// 1M-read launches against 3.51e6 with 262,144), but where single searches take seconds (the
+// 3.1 Gbp human-like index, ~2e5 reads/s) a few large launches leave their tails unshared (peak
+// 6.7e4 with 1M-read launches against 2.0e5 with 262,144).
+constexpr double LAUNCH_SECONDS = 0.5;
+constexpr int LAUNCH_LO = 262144, LAUNCH_HI = 1 << 20;
+
 // Choose the reads of the next launch (caller holds the lock).
 void schedule(galn_hip_server *S, std::vector<Piece> &pieces)
 {
     pieces.clear();
-    int room = S->cfg.max_launch_reads;
+    int room = S->launch_reads;
     for (Request *q : S->waiting) {
         if (room == 0) break;
         const int take = std::min({q->n - q->next, S->cfg.quantum_reads, room});
@@ -196,6 +210,7 @@ void worker_main(galn_hip_server *S, int slot)
             if (S->waiting.empty()) return;  // stop requested and nothing left
             schedule(S, pieces);
             launch_no = ++S->n_launch;
+            if (S->busy_slots == 0) S->busy_mark = Clock::now();
             S->busy_slots += 1;
             S->stats[4] = std::max(S->stats[4], (double)S->busy_slots);
         }
@@ -304,6 +319,14 @@ void worker_main(galn_hip_server *S, int slot)
             S->stats[0] += 1;
             S->stats[1] += m;
             S->stats[3] += std::chrono::duration<double, std::milli>(t1 - t0).count();
+            // recent throughput: each finished launch halves the weight of the ones before it
+            S->rate_reads = 0.5 * S->rate_reads + m;
+            S->rate_sec = 0.5 * S->rate_sec + std::chrono::duration<double>(t1 - S->busy_mark).count();
+            S->busy_mark = t1;
+            if (S->launch_lo < S->launch_hi && S->rate_sec > 0)
+                S->launch_reads = (int)std::clamp(S->rate_reads / S->rate_sec * LAUNCH_SECONDS,
+                                                  (double)S->launch_lo, (double)S->launch_hi);
+            S->stats[6] = S->launch_reads;
         }
         S->cv_done.notify_all();
     }
@@ -320,13 +343,17 @@ int galn_hip_server_open(galn_hip_index *idx, const galn_hip_server_cfg_t *cfg, 
     galn_hip_server *S = new galn_hip_server;
     S->idx = idx;
     if (cfg) S->cfg = *cfg;
-    // 1M-read launches: a launch waits for its longest searches, and with the slots overlapping
-    // larger launches waste less of the device on those tails (100 Mbp index: peak 4.70e6 reads/s
-    // against 3.51e6 at 262,144, p99 at half load 328 ms against 326); small requests still ride
-    // along with large ones (each waiting request gets quantum_reads of every launch first)
-    if (S->cfg.max_launch_reads <= 0) S->cfg.max_launch_reads = 1 << 20;
+    // launch size: fixed when configured, else adaptive (starting at the smallest)
+    if (S->cfg.max_launch_reads > 0) {
+        S->launch_lo = S->launch_hi = S->cfg.max_launch_reads;
+    } else {
+        S->launch_lo = LAUNCH_LO;
+        S->launch_hi = S->cfg.max_launch_reads = LAUNCH_HI;
+    }
+    S->launch_reads = S->launch_lo;
+    S->stats[6] = S->launch_reads;
     if (S->cfg.quantum_reads <= 0) S->cfg.quantum_reads = 16384;
-    S->cfg.quantum_reads = std::min(S->cfg.quantum_reads, S->cfg.max_launch_reads);
+    S->cfg.quantum_reads = std::min(S->cfg.quantum_reads, S->launch_lo);
     if (S->cfg.max_queued_bytes == 0) S->cfg.max_queued_bytes = 4ull << 30;
     if (S->cfg.form != 0) S->cfg.form = 1;
     if (S->cfg.slots <= 0) S->cfg.slots = 3;

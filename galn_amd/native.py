@@ -599,7 +599,7 @@ class Server:
         v = np.zeros(8, np.float64)
         hip_lib().galn_hip_server_stats(self.h, _p(v))
         return {"launches": int(v[0]), "reads": int(v[1]), "requests": int(v[2]), "launch_ms": float(v[3]),
-                "most_in_flight": int(v[4]), "slots": int(v[5])}
+                "most_in_flight": int(v[4]), "slots": int(v[5]), "launch_reads": int(v[6])}
 
     def close(self):
         if self.h:

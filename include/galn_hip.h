@@ -131,7 +131,10 @@ typedef struct {
 
 /* Serving loop configuration (0 = default). */
 typedef struct {
-    int32_t max_launch_reads;  /* reads searched together in one launch (default 1048576) */
+    int32_t max_launch_reads;  /* reads searched together in one launch; 0 (default): adaptive,
+                                  the reads the server finished in its last ~0.5 s of busy time,
+                                  from 262144 to 1048576 (larger launches serve more reads/s where
+                                  no single search is long, smaller ones where some take seconds) */
     int32_t quantum_reads;     /* reads a request gets per launch before the next request's
                                   reads are taken (default 16384) */
     uint64_t max_queued_bytes; /* admission: device bytes of the requests not yet collected
@@ -243,7 +246,8 @@ int galn_hip_server_submit(galn_hip_server *srv, const galn_hip_reads *reads, ui
 int galn_hip_server_wait(galn_hip_server *srv, uint64_t id, int timeout_ms);
 int galn_hip_server_result(galn_hip_server *srv, uint64_t id, galn_hip_out *out, double *latency_ms);
 /* Counters since open: launches, reads searched, requests finished, summed launch ms (host
- * clock around each launch), most launches in flight at once, slots, 0, 0. */
+ * clock around each launch), most launches in flight at once, slots, the current launch size
+ * (reads), 0. */
 int galn_hip_server_stats(galn_hip_server *srv, double out[8]);
 /* Finishes the requests already submitted, then stops the worker. */
 int galn_hip_server_close(galn_hip_server *srv);

@@ -72,6 +72,7 @@ def test_requests_in_a_mix_equal_alone_and_reference(slots, heavy_pops, golden_d
         assert st["requests"] == len(reqs) and st["reads"] == sum(hi - lo for _, lo, hi, _ in reqs)
         assert st["launches"] >= st["reads"] // 512
         assert st["slots"] == slots and 1 <= st["most_in_flight"] <= slots
+        assert st["launch_reads"] == 512  # a configured launch size stays fixed
     for (key, lo, hi, _), a, (g, lat) in zip(reqs, alone, got):
         assert lat >= 0.0
         np.testing.assert_array_equal(g.n_aln, a.n_aln)
@@ -100,6 +101,8 @@ def test_admission_limit_and_release(golden_dir):
             b = srv.submit(*small)
             assert b is not None and srv.wait(b, 60000)
             rb, _ = srv.result(b)
+            # the default launch size adapts within its bounds
+            assert 262144 <= srv.stats()["launch_reads"] <= 1 << 20
     assert ra.n_aln.size == 1490 and rb.n_aln.size == 1
 
 
