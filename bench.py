@@ -351,7 +351,7 @@ def main() -> int:
     ap.add_argument("--serve-seconds", type=float, default=8.0, help="arrival time per offered load (0: no serving)")
     ap.add_argument("--serve-max-request", type=int, default=1_000_000)
     ap.add_argument("--serve-slots", type=int, default=0, help="serving launches on the device at once (0: library default)")
-    ap.add_argument("--serve-launch-reads", type=int, default=0, help="most reads per serving launch (0: library default)")
+    ap.add_argument("--serve-launch-reads", type=int, default=0, help="reads per serving launch (0: the library's adaptive size)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="also copy a buffer of this many MiB device to device three times (the known-size "
                          "kernel that scales the FETCH_SIZE / WRITE_SIZE counters)")

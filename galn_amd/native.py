@@ -535,7 +535,9 @@ class ReadSet:
 
 class Server:
     """The serving loop (galn_hip_server_*): the index stays resident; requests are queued
-    and searched in shared launches, up to `slots` launches on the device at once."""
+    and searched in shared launches, up to `slots` launches on the device at once.
+    `max_launch_reads` 0 sizes launches from the server's recent throughput (262,144 to
+    1,048,576 reads); a positive value fixes it."""
 
     def __init__(self, index: Index, max_launch_reads: int = 0, quantum_reads: int = 0, max_queued_bytes: int = 0,
                  flags: int = 0, form: int = FORM_BATCHED, slots: int = 0, heavy_pops: int = 0):
