@@ -2691,6 +2691,52 @@ int prepare_hand(galn_hip_readset *rs)
     return 0;
 }
 
+int heavy_waves(galn_hip_readset *rs, uint32_t flags);
+
+// The batched form's buffers sized for launches of up to n reads of up to max_len bases: the
+// hand-off slots, the stack pool and wave-engine scratch of min(grid, n) waves, the results' scan
+// space and two staged records per read. Without it these grow by doubling as launches get larger
+// (a server slot's launches vary in size), and each regrowth's hipFree waits for every stream of
+// the device: on the 100 Mbp index a slot regrew ~30 times over its first 50 requests, frees of up
+// to 190 ms, each holding the slot until the other slots' launches ended. Best effort: what does
+// not fit is left to the searches, which allocate what they need.
+int reserve_search(galn_hip_readset *rs, int n, int max_len, uint32_t flags)
+{
+    SearchCtx *cx = rs->ctx;
+    hipStream_t s = cx->stream;
+    n = std::max(n, 1);
+    auto soft = [](int r) {
+        if (r) (void)hipGetLastError(), g_err[0] = 0;
+        return r;
+    };
+    // hand-off slots (seed widths of seeds up to 32 bases included), within prepare_hand's limit
+    const uint64_t stride = (uint64_t)(1 + (max_len + 1) + 33);
+    size_t freeb = 0, totalb = 0;
+    HIP_OK(hipMemGetInfo(&freeb, &totalb));
+    const uint64_t limit = std::min<uint64_t>(HAND_BYTES, (uint64_t)(freeb + rs->hand_bytes) / 4);
+    const uint64_t hand = std::min<uint64_t>((uint64_t)n, limit / (stride * 8)) * stride * 8;
+    if (hand > rs->hand_bytes && soft(grow_dev((void **)&rs->d_hand, rs->hand_bytes, hand, s))) {
+        rs->d_hand = nullptr;
+        rs->hand_bytes = 0;
+        rs->hand_cap = 0;
+    }
+    // the stack pool and the wave engine's per-wave scratch, as prepare_heavy sizes them
+    const int waves = std::max(1, std::min(heavy_waves(rs, flags), n));
+    uint64_t p2 = 1024;
+    while (p2 < (uint64_t)waves * 256) p2 <<= 1;
+    if (!soft(ensure_pool(cx, p2))) {
+        const HLayout L = make_hlayout(256, (int)std::min<uint64_t>(cx->n_gran, 32768));
+        soft(grow_dev((void **)&rs->d_hscratch, rs->hscratch_bytes, (size_t)waves * L.slot_bytes, s));
+    }
+    // the results: the record counts' scan over n reads and the staged records
+    size_t tmp = 0;
+    const rocprim::transform_iterator<const int32_t *, RecCount, uint64_t> cnt(rs->d_out_n, RecCount());
+    HIP_OK(rocprim::inclusive_scan(nullptr, tmp, cnt, rs->d_incl, (size_t)n, rocprim::plus<uint64_t>(), s));
+    soft(grow_dev(&rs->d_scan_tmp, rs->scan_tmp_bytes, tmp, s));
+    soft(grow_dev((void **)&rs->d_stage, rs->stage_bytes, (size_t)2 * n * sizeof(galn_aln_rec_t), s));
+    return 0;
+}
+
 // What a captured search graph holds: when any of it changes, the graph is captured again.
 void fill_key_ptrs(const galn_hip_readset *rs, GraphKey &k)
 {
@@ -3296,6 +3342,16 @@ __attribute__((visibility("hidden"))) int galn_hip_readset_reserve(galn_hip_read
 {
     HIP_OK(hipSetDevice(rs->idx->device));
     return readset_reserve(rs, std::max(n, 1), qbytes, std::max(nb, 1));
+}
+
+// The search buffers of launches of up to n reads of up to max_len bases (the read set's arrays
+// must already hold n reads: galn_hip_readset_reserve).
+__attribute__((visibility("hidden"))) int galn_hip_readset_reserve_search(galn_hip_readset *rs, int n, int max_len,
+                                                                          uint32_t flags)
+{
+    HIP_OK(hipSetDevice(rs->idx->device));
+    if (n > rs->cap_n) return fail(GALN_HIP_E_ARG, "readset_reserve_search: %d reads, the arrays hold %d", n, rs->cap_n);
+    return reserve_search(rs, n, max_len, flags);
 }
 
 namespace {

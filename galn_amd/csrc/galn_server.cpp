@@ -84,6 +84,7 @@ void galn_hip_index_serve(galn_hip_index *idx, int delta);
 void galn_hip_server_worker_mark();
 int galn_hip_readset_make_private(galn_hip_readset *rs);
 int galn_hip_readset_reserve(galn_hip_readset *rs, int n, uint64_t qbytes, int nb);
+int galn_hip_readset_reserve_search(galn_hip_readset *rs, int n, int max_len, uint32_t flags);
 
 namespace {
 
@@ -262,6 +263,9 @@ void worker_main(galn_hip_server *S, int slot)
                 r = galn_hip_readset_reserve(rs, S->cfg.max_launch_reads, (uint64_t)S->cfg.max_launch_reads * 256,
                                              std::max((int)bopts.size(), 1024));
                 if (!r) r = galn_hip_readset_reload(rs, &R);
+                // and the search's buffers for a full launch, so that they are not regrown (each
+                // hipFree waits for the other slots' launches) as launches get larger
+                if (!r) r = galn_hip_readset_reserve_search(rs, S->cfg.max_launch_reads, 256, S->cfg.flags);
             }
         }
         if (!r) {
