@@ -28,7 +28,9 @@ HIP_FLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",
              # no packed-FP32 instructions anywhere in the device code
              "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
              "-Wno-unused-result", "-Wno-unused-value"]
-HOST_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-ffp-contract=off", "-Wall", "-Wno-unused-result"]
+# -mpopcnt: the CPU engine's occurrence counts are popcounts (without it each one is a libgcc call);
+# every x86-64 CPU since 2008 has the instruction
+HOST_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-ffp-contract=off", "-mpopcnt", "-Wall", "-Wno-unused-result"]
 
 
 def _run(cmd: list[str], quiet_feature_warning: bool = False) -> None:

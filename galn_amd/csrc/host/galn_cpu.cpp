@@ -47,29 +47,50 @@ struct Idx {
     uint64_t L2[5];
 };
 
-// counts of base c among the first t (0..16) symbols of word w, one per 2-bit field
-inline uint32_t word_count(uint32_t w, int c, int t)
+// The 8 symbol words of a block are read as 4 chunks of 64 bits (words 2q and 2q + 1, symbol j of
+// the chunk in bits 63-2j and 62-2j): one popcount round counts 32 symbols.
+constexpr uint64_t M55 = 0x5555555555555555ull;
+
+inline uint64_t chunk_of(const uint32_t *w, int q) { return (uint64_t)w[2 * q] << 32 | w[2 * q + 1]; }
+
+// the symbol fields of symbols [a, b) of a chunk (0 <= a < b <= 32), as the low bit of each field
+inline uint64_t span_mask(int a, int b)
 {
-    if (t <= 0) return 0;
-    const uint32_t hi = (w >> 1) & 0x55555555u, lo = w & 0x55555555u;
-    const uint32_t x = ((c & 2) ? hi : ~hi) & ((c & 1) ? lo : ~lo) & 0x55555555u;
-    const uint32_t keep = t >= 16 ? 0xffffffffu : ~(0xffffffffu >> (2 * t));
-    return (uint32_t)__builtin_popcount(x & keep);
+    uint64_t keep = M55;
+    if (b < 32) keep &= ~(~0ull >> (2 * b));
+    if (a > 0) keep &= ~0ull >> (2 * a);
+    return keep;
 }
 
-// all four counts among the first t symbols of w
-inline void word_count4(uint32_t w, int t, uint32_t n[4])
+// adds the counts of the four bases among symbols [s0, s1) of a block's words w (0 <= s0 <= s1 <= 128)
+inline void range_count4(const uint32_t *w, int s0, int s1, uint64_t n[4])
 {
-    if (t <= 0) return;
-    const uint32_t keep = (t >= 16 ? 0xffffffffu : ~(0xffffffffu >> (2 * t))) & 0x55555555u;
-    const uint32_t hi = (w >> 1) & keep, lo = w & keep;
-    const uint32_t n3 = (uint32_t)__builtin_popcount(hi & lo);
-    const uint32_t n2 = (uint32_t)__builtin_popcount(hi) - n3;
-    const uint32_t n1 = (uint32_t)__builtin_popcount(lo) - n3;
-    n[0] += (uint32_t)t - n1 - n2 - n3;
-    n[1] += n1;
-    n[2] += n2;
-    n[3] += n3;
+    if (s0 >= s1) return;
+    for (int q = s0 >> 5; q <= (s1 - 1) >> 5; ++q) {
+        const int a = std::max(s0 - 32 * q, 0), b = std::min(s1 - 32 * q, 32);
+        const uint64_t keep = span_mask(a, b), x = chunk_of(w, q);
+        const uint64_t hi = (x >> 1) & keep, lo = x & keep;
+        const uint64_t n3 = (uint64_t)__builtin_popcountll(hi & lo);
+        const uint64_t nh = (uint64_t)__builtin_popcountll(hi), nl = (uint64_t)__builtin_popcountll(lo);
+        n[0] += (uint64_t)(b - a) - nh - nl + n3;
+        n[1] += nl - n3;
+        n[2] += nh - n3;
+        n[3] += n3;
+    }
+}
+
+// the count of base c among symbols [s0, s1) of a block's words w
+inline uint64_t range_count1(const uint32_t *w, int s0, int s1, int c)
+{
+    uint64_t n = 0;
+    if (s0 >= s1) return 0;
+    for (int q = s0 >> 5; q <= (s1 - 1) >> 5; ++q) {
+        const int a = std::max(s0 - 32 * q, 0), b = std::min(s1 - 32 * q, 32);
+        const uint64_t x = chunk_of(w, q);
+        const uint64_t hi = x >> 1, lo = x;
+        n += (uint64_t)__builtin_popcountll(((c & 2) ? hi : ~hi) & ((c & 1) ? lo : ~lo) & span_mask(a, b));
+    }
+    return n;
 }
 
 inline const uint32_t *block_of(const Idx &x, uint64_t p) { return x.bwt + ((p >> 7) << 4); }
@@ -79,33 +100,32 @@ inline uint64_t count_before(const uint32_t *blk, int c)
     return (uint64_t)blk[2 * c] | (uint64_t)blk[2 * c + 1] << 32;
 }
 
-// Occ(c, k): occurrences of c in BWT rows 0..k (k = -1: none; k = seq_len: the total)
-uint64_t occ1(const Idx &x, uint64_t k, int c)
+// Occ of the four bases at a and b (a < b; a = -1: none), bwt_2occ4: when both rows are in one
+// block the second count continues from the first
+void occ4_pair(const Idx &x, uint64_t a, uint64_t b, uint64_t ca[4], uint64_t cb[4])
 {
-    if (k == ~0ull) return 0;
-    if (k == x.seq_len) return x.L2[c + 1] - x.L2[c];
-    const uint64_t p = k - (k >= x.primary);
-    const uint32_t *b = block_of(x, p);
-    uint64_t n = count_before(b, c);
-    const int r = (int)(p & 127) + 1;  // symbols of the block up to p
-    const uint32_t *w = b + 8;
-    for (int j = 0; j < 8 && 16 * j < r; ++j) n += word_count(w[j], c, std::min(16, r - 16 * j));
-    return n;
-}
-
-void occ4(const Idx &x, uint64_t k, uint64_t cnt[4])
-{
-    if (k == ~0ull) {
-        cnt[0] = cnt[1] = cnt[2] = cnt[3] = 0;
-        return;
+    const uint64_t bp = b - (b >= x.primary);
+    const uint32_t *bb = block_of(x, bp);
+    const int rb = (int)(bp & 127) + 1;  // symbols of the block up to bp
+    if (a == ~0ull) {
+        ca[0] = ca[1] = ca[2] = ca[3] = 0;
+    } else {
+        const uint64_t ap = a - (a >= x.primary);
+        const uint32_t *ab = block_of(x, ap);
+        const int ra = (int)(ap & 127) + 1;
+        uint64_t n[4] = {0, 0, 0, 0};
+        range_count4(ab + 8, 0, ra, n);
+        for (int c = 0; c < 4; ++c) ca[c] = count_before(ab, c) + n[c];
+        if (ab == bb) {
+            uint64_t d[4] = {0, 0, 0, 0};
+            range_count4(bb + 8, ra, rb, d);
+            for (int c = 0; c < 4; ++c) cb[c] = ca[c] + d[c];
+            return;
+        }
     }
-    const uint64_t p = k - (k >= x.primary);
-    const uint32_t *b = block_of(x, p);
-    const int r = (int)(p & 127) + 1;
-    uint32_t n[4] = {0, 0, 0, 0};
-    const uint32_t *w = b + 8;
-    for (int j = 0; j < 8 && 16 * j < r; ++j) word_count4(w[j], std::min(16, r - 16 * j), n);
-    for (int c = 0; c < 4; ++c) cnt[c] = count_before(b, c) + n[c];
+    uint64_t n[4] = {0, 0, 0, 0};
+    range_count4(bb + 8, 0, rb, n);
+    for (int c = 0; c < 4; ++c) cb[c] = count_before(bb, c) + n[c];
 }
 
 // algorithmic bytes of a lookup at (a, b): 64 per distinct block; position -1 needs none
@@ -122,20 +142,20 @@ inline uint32_t lookup_bytes(const Idx &x, uint64_t a, uint64_t b)
 inline void occ1_pair(const Idx &x, uint64_t a, uint64_t b, int c, uint64_t &oa, uint64_t &ob)
 {
     const uint64_t bp = b - (b >= x.primary);
+    const uint32_t *bb = block_of(x, bp);
+    const int rb = (int)(bp & 127) + 1;
     oa = 0;
     if (a != ~0ull) {
         const uint64_t ap = a - (a >= x.primary);
-        const uint32_t *blk = block_of(x, ap);
-        uint64_t n = count_before(blk, c);
-        const int r = (int)(ap & 127) + 1;
-        for (int j = 0; j < 8 && 16 * j < r; ++j) n += word_count(blk[8 + j], c, std::min(16, r - 16 * j));
-        oa = n;
+        const uint32_t *ab = block_of(x, ap);
+        const int ra = (int)(ap & 127) + 1;
+        oa = count_before(ab, c) + range_count1(ab + 8, 0, ra, c);
+        if (ab == bb) {
+            ob = oa + range_count1(bb + 8, ra, rb, c);
+            return;
+        }
     }
-    const uint32_t *blk = block_of(x, bp);
-    uint64_t n = count_before(blk, c);
-    const int r = (int)(bp & 127) + 1;
-    for (int j = 0; j < 8 && 16 * j < r; ++j) n += word_count(blk[8 + j], c, std::min(16, r - 16 * j));
-    ob = n;
+    ob = count_before(bb, c) + range_count1(bb + 8, 0, rb, c);
 }
 
 inline void prefetch_lookup(const Idx &x, uint64_t a, uint64_t b)
@@ -224,6 +244,7 @@ public:
     int result = 0;  // n_aln, or a negative code
     std::vector<galn_aln_rec_t> recs;
     uint64_t pops = 0, trace = TR_INIT, bytes = 0;
+    bool want_trace = true;  // hash the pops (the caller asked for traces)
 
 private:
     enum { PH_WIDTH, PH_LOOP, PH_EXACT, PH_EXPAND, PH_DONE };
@@ -335,6 +356,7 @@ private:
         if (b.empty()) nonempty[wi] &= ~(1ull << (s & 63));
         --n_entries;
         ++pops;
+        if (!want_trace) return;
         trace = tr_mix(trace, e.info);
         trace = tr_mix(trace, e.k);
         trace = tr_mix(trace, e.l);
@@ -524,8 +546,7 @@ private:
         phase = PH_LOOP;
         const int i = ei - 1;
         uint64_t ck[4], cl[4];
-        occ4(*X, e.k - 1, ck);
-        occ4(*X, e.l, cl);
+        occ4_pair(*X, e.k - 1, e.l, ck, cl);
         bytes += lookup_bytes(*X, e.k - 1, e.l);
         const uint64_t occ = e.l - e.k + 1;
         bool allow_diff = true, allow_M = true;
@@ -624,6 +645,7 @@ extern "C" int galn_cpu_search(const uint32_t *bwt, const galn_bwt_meta_t *meta,
     std::vector<std::vector<ReadOut>> touts((size_t)n_threads);
     auto worker = [&](int tid) {
         std::vector<Search> slot((size_t)interleave);
+        for (Search &S : slot) S.want_trace = out->trace != nullptr;
         std::vector<int> rid((size_t)interleave, -1);
         int lo = 0, hi = 0;  // claimed positions not yet started
         auto start = [&](int s) -> bool {

@@ -808,7 +808,8 @@ static int cpu_run(const uint32_t *bwt, const galn_bwt_meta_t *meta, const galn_
     const int n = list ? (int)list->size() : R->n_reads;
     so.resize(n);
     for (;;) {
-        galn_hip_out O = so.view(true);
+        // pop traces only for --trace output and --verify (hashing every pop costs the engine time)
+        galn_hip_out O = so.view(A.trace || A.verify);
         const int r = galn_cpu_search(bwt, meta, R, list ? list->data() : nullptr, n, &O, A.cpu_threads, A.cpu_interleave);
         if (r == GALN_HIP_E_RECCAP) {
             so.rec.resize(O.n_rec);
