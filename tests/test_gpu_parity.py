@@ -130,6 +130,21 @@ def test_cli_streams_blocks_of_one_batch(key, golden_dir, tmp_path):
     assert fx.md5_file(str(out)) == w["meta"]["runs"][key]["sai_md5"]
 
 
+@pytest.mark.parametrize("name,key", [("adna_mixed", "adna_1"), ("plain_small", "default_1")])
+def test_cli_verify_reruns_reads_on_cpu_engine(name, key, golden_dir, tmp_path):
+    """`galn aln --verify` re-runs every GPU read on the CPU engine: records, pops and pop traces
+    agree, and the .sai is still the reference's."""
+    print(f"case cli verify {name}/{key}", flush=True)
+    w = golden_inputs(name, golden_dir)
+    opts, mate = run_opts(w["meta"], key)
+    out = tmp_path / "out.sai"
+    r = subprocess.run([nv.GALN_BIN, "aln", "--verify", *opts, "-f", str(out), w["prefix"], w["reads"][mate]],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    assert "differ -> PASS" in r.stderr, r.stderr
+    assert fx.md5_file(str(out)) == w["meta"]["runs"][key]["sai_md5"]
+
+
 def test_cli_streamed_equals_whole_input_on_many_batches(golden_dir, tmp_path):
     """300k reads (two bwa batches, gzip input): streamed in blocks of one batch, of four, and
     read whole first, the .sai bytes are the same."""
