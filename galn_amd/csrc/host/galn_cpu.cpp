@@ -62,21 +62,27 @@ inline uint64_t span_mask(int a, int b)
     return keep;
 }
 
-// adds the counts of the four bases among symbols [s0, s1) of a block's words w (0 <= s0 <= s1 <= 128)
+// adds the counts of the four bases among symbols [s0, s1) of a block's words w (0 <= s0 <= s1 <= 128):
+// popcounts of the high bits, the low bits and both, summed over the chunks (only the first and the
+// last chunk of the range need a mask), then split into the four bases once
 inline void range_count4(const uint32_t *w, int s0, int s1, uint64_t n[4])
 {
     if (s0 >= s1) return;
-    for (int q = s0 >> 5; q <= (s1 - 1) >> 5; ++q) {
-        const int a = std::max(s0 - 32 * q, 0), b = std::min(s1 - 32 * q, 32);
-        const uint64_t keep = span_mask(a, b), x = chunk_of(w, q);
+    const int q0 = s0 >> 5, q1 = (s1 - 1) >> 5;
+    uint64_t nh = 0, nl = 0, nb = 0;
+    for (int q = q0; q <= q1; ++q) {
+        uint64_t keep = M55;
+        if (q == q0 || q == q1) keep = span_mask(q == q0 ? s0 - 32 * q : 0, q == q1 ? s1 - 32 * q : 32);
+        const uint64_t x = chunk_of(w, q);
         const uint64_t hi = (x >> 1) & keep, lo = x & keep;
-        const uint64_t n3 = (uint64_t)__builtin_popcountll(hi & lo);
-        const uint64_t nh = (uint64_t)__builtin_popcountll(hi), nl = (uint64_t)__builtin_popcountll(lo);
-        n[0] += (uint64_t)(b - a) - nh - nl + n3;
-        n[1] += nl - n3;
-        n[2] += nh - n3;
-        n[3] += n3;
+        nb += (uint64_t)__builtin_popcountll(hi & lo);
+        nh += (uint64_t)__builtin_popcountll(hi);
+        nl += (uint64_t)__builtin_popcountll(lo);
     }
+    n[0] += (uint64_t)(s1 - s0) - nh - nl + nb;
+    n[1] += nl - nb;
+    n[2] += nh - nb;
+    n[3] += nb;
 }
 
 // the count of base c among symbols [s0, s1) of a block's words w
@@ -84,11 +90,14 @@ inline uint64_t range_count1(const uint32_t *w, int s0, int s1, int c)
 {
     uint64_t n = 0;
     if (s0 >= s1) return 0;
-    for (int q = s0 >> 5; q <= (s1 - 1) >> 5; ++q) {
-        const int a = std::max(s0 - 32 * q, 0), b = std::min(s1 - 32 * q, 32);
-        const uint64_t x = chunk_of(w, q);
-        const uint64_t hi = x >> 1, lo = x;
-        n += (uint64_t)__builtin_popcountll(((c & 2) ? hi : ~hi) & ((c & 1) ? lo : ~lo) & span_mask(a, b));
+    const int q0 = s0 >> 5, q1 = (s1 - 1) >> 5;
+    // the symbols equal to c, as the low bit of each field: x ^ ~c repeated has both bits set there
+    const uint64_t pat = ~((uint64_t)(c & 3) * M55);
+    for (int q = q0; q <= q1; ++q) {
+        uint64_t keep = M55;
+        if (q == q0 || q == q1) keep = span_mask(q == q0 ? s0 - 32 * q : 0, q == q1 ? s1 - 32 * q : 32);
+        const uint64_t y = chunk_of(w, q) ^ pat;
+        n += (uint64_t)__builtin_popcountll(y & (y >> 1) & keep);
     }
     return n;
 }
