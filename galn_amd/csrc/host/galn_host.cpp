@@ -763,7 +763,7 @@ static int usage()
             "         --gpu-only         the GPU searches every read it can; the CPU only those outside\n"
             "                            the GPU kernels' limits (the default here)\n"
             "         --cpu-workers INT  CPU search threads [all hardware threads]\n"
-            "         --cpu-interleave INT  reads one CPU thread advances in turn [8]\n"
+            "         --cpu-interleave INT  reads one CPU thread advances in turn [2, 4 or 8 by index size]\n"
             "         --verify           re-run every read on the CPU and compare (exit 6 on a difference)\n"
             "         --block-batches INT  stream the input to the GPU in blocks of INT bwa batches of\n"
             "                            0x40000 reads, read, searched and written at the same time;\n"

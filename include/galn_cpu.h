@@ -22,9 +22,10 @@ extern "C" {
 /* Searches reads (all of them, or the n_list read ids in `list`) against the .bwt body `bwt`
  * (host memory, the layout galn_load_bwt returns). Results go to `out` indexed by position:
  * read i, or list[i] at position i. Records are in position order; per-read codes as the GPU's
- * (GALN_READ_ERR_SCORE for a push outside [0, n_stacks)); n_pops, trace (always computed) and
- * occ_bytes are filled when given. n_threads <= 0: every hardware thread; interleave: reads one
- * thread advances in turn (<= 0: 8). Returns 0, GALN_HIP_E_ARG, or GALN_HIP_E_RECCAP with
+ * (GALN_READ_ERR_SCORE for a push outside [0, n_stacks)); n_pops, trace and occ_bytes are
+ * filled when given (the trace hash is computed only then). n_threads <= 0: every hardware
+ * thread; interleave: reads one thread advances in turn (<= 0: by the .bwt body's size, 2 under
+ * 8 MiB, 4 under 32 MiB, else 8). Returns 0, GALN_HIP_E_ARG, or GALN_HIP_E_RECCAP with
  * out->n_rec = records needed. */
 int galn_cpu_search(const uint32_t *bwt, const galn_bwt_meta_t *meta, const galn_hip_reads *reads,
                     const int32_t *list, int32_t n_list, galn_hip_out *out, int32_t n_threads, int32_t interleave);

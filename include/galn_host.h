@@ -60,7 +60,8 @@ typedef struct {
     int verify;             /* --verify: re-run the GPU's reads on the CPU engine and compare
                                records, pops and traces (exit 6 on a difference) */
     int cpu_threads;        /* --cpu-workers N: CPU engine threads [all hardware threads] */
-    int cpu_interleave;     /* --cpu-interleave N: reads a CPU thread advances in turn [8] */
+    int cpu_interleave;     /* --cpu-interleave N: reads a CPU thread advances in turn [0: 2, 4 or 8
+                               by the index's size] */
     int block_batches;      /* --block-batches N: the GPU path streams the input in blocks of N bwa
                                batches (0x40000 reads each), searched while the next are read and
                                the last written [4]; 0: read the whole input first */

@@ -38,7 +38,7 @@ def test_cpu_engine_matches_reference_golden(name, key, golden_dir):
     assert fx.md5_bytes(nv.sai_bytes(o, res.n_aln, res.rec_off, res.rec)) == w["meta"]["runs"][key]["sai_md5"]
 
 
-@pytest.mark.parametrize("threads,interleave", [(1, 1), (1, 16), (3, 5)])
+@pytest.mark.parametrize("threads,interleave", [(1, 1), (1, 16), (3, 5), (2, 0)])  # 0: chosen by index size
 def test_cpu_engine_independent_of_threads_and_interleave(threads, interleave, golden_dir):
     w = golden_inputs("adna_mixed", golden_dir)
     o, r, res = engine_run(w, "adna_1", threads=threads, interleave=interleave)
