@@ -147,7 +147,10 @@ typedef struct {
                                   stream and stack pool; slot 0 uses the index's (default 3, at
                                   most 16; HIP spreads streams over GPU_MAX_HW_QUEUES hardware
                                   queues, 4 by default, and slots whose streams share one run
-                                  one after the other) */
+                                  one after the other); before its first launch a slot reserves
+                                  the search buffers of a full launch (default launch sizes: a
+                                  16 GiB stack pool, ~2.3 GB of hand-off slots, ~1.2 GB of
+                                  wave-engine scratch) */
     int32_t heavy_pops;        /* batched form: pops after which a read moves to the wave-per-read
                                   engine, as galn_hip_tune_t.heavy_pops (0: the library default) */
 } galn_hip_server_cfg_t;
