@@ -67,7 +67,9 @@ def test_cpu_engine_read_list_and_bytes(golden_dir):
 
 @pytest.mark.parametrize("name,key", [("plain_small", "default_1"), ("adna_mixed", "adna_1"), ("deep_repeats", "deep_2"),
                                       ("options", "max_entries_m300_1"), ("options", "nonstop_n2_o0_1"),
-                                      ("fastq_edge", "adna_1"), ("serving_mix", "default_1")])
+                                      ("fastq_edge", "adna_1"), ("serving_mix", "default_1"),
+                                      ("beyond_limits", "default_1"), ("beyond_limits", "e50_1"),
+                                      ("beyond_limits", "n130_1")])
 def test_cli_cpu_only_writes_golden_sai(name, key, golden_dir, tmp_path):
     """`galn aln --cpu-only` writes the reference's .sai with no GPU."""
     import subprocess
